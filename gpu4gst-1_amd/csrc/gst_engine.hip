@@ -1,0 +1,527 @@
+// gst_engine.hip — host side of libgst.so: the C ABI declared in include/gst.h.
+//
+// The graph (CSR + group table) is uploaded once by gst_open and stays resident. A job
+// is one query with its own DP arena and worklists ("slot"). gst_solve runs one job with
+// host-driven rounds: after each round the host reads the job's state (one sync per
+// round, like code/TrimCDP/src/DPQ.cu:512-544) and launches exactly-sized grids.
+// gst_solve_batch and the serving API run many jobs with device-side phase control and
+// the round loop captured in a HIP graph (gst_batch.inc).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/gst.h"
+#include "gst_kernels.hpp"
+
+using namespace gst;
+using clk = std::chrono::steady_clock;
+
+namespace {
+
+thread_local std::string g_err;
+void set_err(const std::string &s) { g_err = s; }
+
+#define HIPCHK(x)                                                                    \
+  do {                                                                               \
+    hipError_t e_ = (x);                                                             \
+    if (e_ != hipSuccess) {                                                          \
+      set_err(std::string(#x) + ": " + hipGetErrorString(e_));                      \
+      return GST_DEVICE_ERROR;                                                       \
+    }                                                                                \
+  } while (0)
+
+double secs(clk::time_point a, clk::time_point b) {
+  return std::chrono::duration<double>(b - a).count();
+}
+
+// Device buffers of one job slot.
+struct Slot {
+  int K = 0, D = -1, H = 1, W = 1;
+  size_t arena_words = 0;
+  int32_t *T = nullptr;
+  unsigned long long *act = nullptr;
+  int32_t *lst = nullptr, *blst = nullptr, *tab = nullptr, *stk = nullptr, *out = nullptr;
+  size_t bytes = 0;
+  int nl = 0;
+};
+
+constexpr int SCAP = 1 << 16;  // rebuild stack entries
+constexpr int ECAP = 1 << 15;  // tree edges returned
+
+}  // namespace
+
+struct gst_ctx {
+  int dev = 0;
+  hipStream_t st = nullptr;
+  int32_t n = 0;
+  int64_t m = 0;
+  GraphDev g{};
+  int32_t *d_beg = nullptr, *d_adj = nullptr, *d_w = nullptr, *d_gverts = nullptr;
+  long long *d_goffs = nullptr;
+  std::vector<int64_t> h_goffs;
+  std::vector<int32_t> h_deg;
+  int32_t n_groups = 0;
+  int64_t graph_bytes = 0;
+  long long bcap = 0;  // wavefront items one worklist can hold
+  gst_opts opts{};
+  int32_t split = 32, chunk = 512;
+  // single-query path
+  std::mutex solve_mu;
+  Slot plain;
+  JobConst *d_jc = nullptr;  // [max_batch]
+  JobState *d_js = nullptr;  // [max_batch]
+  JobState *h_js = nullptr;  // pinned mirror [max_batch]
+  unsigned long long *d_count = nullptr;
+  int max_batch = 64;
+  // batched / served (gst_batch.inc)
+  struct Batch;
+  Batch *batch = nullptr;
+};
+
+namespace {
+
+int64_t arena_bytes_for(int64_t n, int K, int D) {
+  const int64_t H = D >= 0 ? D + 1 : 1;
+  return n * ((int64_t)1 << K) * H * 4;
+}
+
+// bytes of a slot for (K, D)
+size_t slot_bytes(const gst_ctx *c, int K, int D) {
+  const int H = D >= 0 ? D + 1 : 1, W = 1 << K;
+  const int nl = D >= 0 ? H : 2;
+  size_t b = (size_t)c->n * W * H * 4;                 // arena
+  b += (size_t)nl * c->n * 8;                           // act
+  b += (size_t)nl * c->n * 4;                           // thread lists
+  b += (size_t)nl * c->bcap * 8;                        // wavefront lists
+  b += (size_t)(MAXK * MAXK * MAXW + MAXK * MAXW + MAXK * MAXK) * 4;  // tables
+  b += (size_t)SCAP * 12 + (size_t)ECAP * 12;
+  return b;
+}
+
+void slot_free(Slot &s) {
+  if (s.T) (void)hipFree(s.T);
+  s = Slot();
+}
+
+int slot_alloc(gst_ctx *c, Slot &s, int K, int D) {
+  slot_free(s);
+  s.K = K; s.D = D;
+  s.H = D >= 0 ? D + 1 : 1;
+  s.W = 1 << K;
+  s.nl = D >= 0 ? s.H : 2;
+  s.arena_words = (size_t)c->n * s.W * s.H;
+  s.bytes = slot_bytes(c, K, D);
+  char *base = nullptr;
+  hipError_t e = hipMalloc((void **)&base, s.bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    set_err(std::string("hipMalloc(slot) failed: ") + hipGetErrorString(e));
+    s = Slot();
+    return GST_OVER_BUDGET;
+  }
+  size_t off = 0;
+  auto take = [&](size_t bytes) { char *p = base + off; off += (bytes + 255) & ~(size_t)255; return p; };
+  s.T = (int32_t *)take(s.arena_words * 4);
+  s.act = (unsigned long long *)take((size_t)s.nl * c->n * 8);
+  s.lst = (int32_t *)take((size_t)s.nl * c->n * 4);
+  s.blst = (int32_t *)take((size_t)s.nl * c->bcap * 8);
+  s.tab = (int32_t *)take((size_t)(MAXK * MAXK * MAXW + MAXK * MAXW + MAXK * MAXK) * 4);
+  s.stk = (int32_t *)take((size_t)SCAP * 12);
+  s.out = (int32_t *)take((size_t)ECAP * 12);
+  if (off > s.bytes) {  // alignment slack
+    (void)hipFree(base);
+    s.bytes = off + 4096;
+    e = hipMalloc((void **)&base, s.bytes);
+    if (e != hipSuccess) { (void)hipGetLastError(); set_err("hipMalloc(slot) failed"); s = Slot(); return GST_OVER_BUDGET; }
+    off = 0;
+    s.T = (int32_t *)take(s.arena_words * 4);
+    s.act = (unsigned long long *)take((size_t)s.nl * c->n * 8);
+    s.lst = (int32_t *)take((size_t)s.nl * c->n * 4);
+    s.blst = (int32_t *)take((size_t)s.nl * c->bcap * 8);
+    s.tab = (int32_t *)take((size_t)(MAXK * MAXK * MAXW + MAXK * MAXW + MAXK * MAXK) * 4);
+    s.stk = (int32_t *)take((size_t)SCAP * 12);
+    s.out = (int32_t *)take((size_t)ECAP * 12);
+  }
+  return GST_OK;
+}
+
+// Validated, de-duplicated query.
+struct Query {
+  std::vector<int32_t> gids;
+  int K = 0, D = -1, flags = 0;
+};
+
+int check_query(const gst_ctx *c, const int32_t *gids, int32_t k, int32_t D, int32_t flags, Query &q) {
+  if (!gids || k < 1) { set_err("query needs k >= 1 group ids"); return GST_BAD_INPUT; }
+  q.gids.clear();
+  for (int i = 0; i < k; i++) {
+    if (gids[i] < 0 || gids[i] >= c->n_groups) { set_err("group id out of range"); return GST_BAD_INPUT; }
+    if (std::find(q.gids.begin(), q.gids.end(), gids[i]) == q.gids.end()) q.gids.push_back(gids[i]);
+  }
+  q.K = (int)q.gids.size();
+  if (q.K > MAXK) { set_err("k > GST_MAX_K"); return GST_BAD_INPUT; }
+  if (D > GST_MAX_D) { set_err("D > GST_MAX_D"); return GST_BAD_INPUT; }
+  q.D = D < 0 ? -1 : D;
+  q.flags = flags | c->opts.flags;
+  return GST_OK;
+}
+
+void fill_jobconst(const gst_ctx *c, const Slot &s, const Query &q, JobConst &jc) {
+  std::memset(&jc, 0, sizeof(jc));
+  jc.T = s.T; jc.act = s.act; jc.lst = s.lst; jc.blst = s.blst; jc.tab = s.tab;
+  jc.stk = s.stk; jc.out = s.out;
+  jc.bcap = c->bcap; jc.scap = SCAP; jc.ecap = ECAP;
+  jc.K = q.K; jc.D = q.D; jc.H = q.D >= 0 ? q.D + 1 : 1; jc.W = 1 << q.K; jc.R = jc.W - 1;
+  jc.hop = q.D >= 0; jc.flags = q.flags; jc.nl = jc.hop ? jc.H : 2;
+  for (int i = 0; i < q.K; i++) jc.gid[i] = q.gids[i];
+  int cnt[8] = {0};
+  for (int p = 0; p < jc.W; p++) {
+    const int L = __builtin_popcount(p);
+    jc.rank[p] = (uint8_t)cnt[L];
+    jc.sub[L][cnt[L]++] = (uint8_t)p;
+  }
+  for (int L = 0; L <= q.K; L++) jc.nsub[L] = (uint8_t)cnt[L];
+}
+
+void init_jobstate(JobState &js) {
+  std::memset(&js, 0, sizeof(js));
+  js.phase = PH_INIT;
+  js.ub = INF;
+  js.r1 = LB_CAP;
+  js.ans = ~0ull;
+}
+
+inline unsigned grid_for(long long items, long long per_block, unsigned cap) {
+  long long b = (items + per_block - 1) / per_block;
+  if (b < 1) b = 1;
+  if (b > cap) b = cap;
+  return (unsigned)b;
+}
+
+// Launch the init kernels of a job (arena = INF, masks = 0) on stream st.
+void launch_job_init(hipStream_t st, const Slot &s, const gst_ctx *c) {
+  k_fill_i32<<<grid_for((long long)s.arena_words / 4, BLK, 8192), BLK, 0, st>>>(s.T, (long long)s.arena_words, INF);
+  k_fill_u64<<<grid_for((long long)s.nl * c->n, BLK, 4096), BLK, 0, st>>>(s.act, (long long)s.nl * c->n, 0ull);
+}
+
+void finish_result(const gst_ctx *c, const Query &q, const JobState &js, const std::vector<int32_t> &edges,
+                   gst_result *out) {
+  std::memset(out, 0, sizeof(*out));
+  out->k = q.K;
+  out->D = q.D;
+  out->rounds = js.rounds;
+  out->states_processed = (int64_t)js.st[ST_STATES];
+  out->grow_vertices = (int64_t)js.st[ST_VERTS];
+  out->grow_edges = (int64_t)js.st[ST_EDGES];
+  out->grow_relax = (int64_t)js.st[ST_RELAX];
+  out->grow_pushes = (int64_t)js.st[ST_PUSH];
+  out->arena_bytes = arena_bytes_for(c->n, q.K, q.D);
+  out->root = -1;
+  out->root_height = -1;
+  if (js.err) {
+    out->status = GST_INTERNAL;
+    out->cost = -1;
+    char buf[64];
+    std::snprintf(buf, sizeof buf, "tree reconstruction failed (code %d)", js.err);
+    set_err(buf);
+    return;
+  }
+  if (js.ans == ~0ull) {
+    out->status = GST_NO_SOLUTION;
+    out->cost = -1;
+    return;
+  }
+  const int H = q.D >= 0 ? q.D + 1 : 1;
+  const long long idx = (long long)(js.ans & 0xffffffffull);
+  out->status = GST_OK;
+  out->cost = (int32_t)(js.ans >> 32);
+  out->root = (int32_t)(idx / H);
+  out->root_height = q.D >= 0 ? (int32_t)(idx % H) : -1;
+  // distinct edges, sorted
+  std::vector<std::array<int32_t, 3>> e;
+  for (size_t i = 0; i + 2 < edges.size(); i += 3)
+    e.push_back({edges[i], edges[i + 1], edges[i + 2]});
+  std::sort(e.begin(), e.end());
+  e.erase(std::unique(e.begin(), e.end(), [](const std::array<int32_t, 3> &a, const std::array<int32_t, 3> &b) {
+            return a[0] == b[0] && a[1] == b[1];
+          }), e.end());
+  out->n_edges = (int32_t)e.size();
+  long long sum = 0;
+  if (!e.empty()) {
+    out->edges = (int32_t *)std::malloc(sizeof(int32_t) * 3 * e.size());
+    for (size_t i = 0; i < e.size(); i++) {
+      out->edges[3 * i] = e[i][0]; out->edges[3 * i + 1] = e[i][1]; out->edges[3 * i + 2] = e[i][2];
+      sum += e[i][2];
+    }
+  }
+  out->tree_cost = (int32_t)sum;
+}
+
+}  // namespace
+
+#include "gst_batch.inc"
+
+extern "C" {
+
+const char *gst_last_error(void) { return g_err.c_str(); }
+
+const char *gst_strerror(int code) {
+  switch (code) {
+    case GST_OK: return "ok";
+    case GST_NO_SOLUTION: return "no feasible tree (a query group cannot be connected)";
+    case GST_PENDING: return "pending";
+    case GST_BAD_INPUT: return "bad input";
+    case GST_OVER_BUDGET: return "DP arena does not fit the memory budget";
+    case GST_DEVICE_ERROR: return "device error";
+    case GST_INTERNAL: return "internal error";
+    case GST_NO_DEVICE: return "no HIP device";
+    default: return "unknown status";
+  }
+}
+
+int64_t gst_arena_bytes(int32_t n, int32_t k, int32_t D) { return arena_bytes_for(n, k, D); }
+
+int gst_open(const gst_csr *csr, const gst_groups *groups, const gst_opts *opts, gst_ctx **out) {
+  if (!csr || !groups || !out) { set_err("null argument"); return GST_BAD_INPUT; }
+  *out = nullptr;
+  const int32_t n = csr->n;
+  const int64_t m = csr->m;
+  if (n < 1 || m < 0 || m >= (1ll << 31) || !csr->beg_pos || (m && (!csr->adj || !csr->w))) {
+    set_err("bad CSR sizes (need n >= 1, m < 2^31)");
+    return GST_BAD_INPUT;
+  }
+  if (csr->beg_pos[0] != 0 || csr->beg_pos[n] != m) { set_err("beg_pos must run from 0 to m"); return GST_BAD_INPUT; }
+  for (int32_t v = 0; v < n; v++)
+    if (csr->beg_pos[v + 1] < csr->beg_pos[v]) { set_err("beg_pos not monotone"); return GST_BAD_INPUT; }
+  for (int64_t e = 0; e < m; e++) {
+    if (csr->adj[e] < 0 || csr->adj[e] >= n) { set_err("adjacency id out of range"); return GST_BAD_INPUT; }
+    if (csr->w[e] < 1 || csr->w[e] > (1 << 20)) { set_err("edge weight outside [1, 2^20]"); return GST_BAD_INPUT; }
+  }
+  if (groups->n_groups < 0 || (groups->n_groups > 0 && (!groups->offs || !groups->verts))) {
+    set_err("bad group table"); return GST_BAD_INPUT;
+  }
+  const int64_t total = groups->n_groups ? groups->offs[groups->n_groups] : 0;
+  if (groups->n_groups && groups->offs[0] != 0) { set_err("group offsets must start at 0"); return GST_BAD_INPUT; }
+  for (int32_t gI = 0; gI < groups->n_groups; gI++) {
+    if (groups->offs[gI + 1] < groups->offs[gI]) { set_err("group offsets not monotone"); return GST_BAD_INPUT; }
+    for (int64_t t = groups->offs[gI]; t < groups->offs[gI + 1]; t++) {
+      if (groups->verts[t] < 0 || groups->verts[t] >= n) { set_err("group member out of range"); return GST_BAD_INPUT; }
+      if (t > groups->offs[gI] && groups->verts[t] < groups->verts[t - 1]) { set_err("group members must be ascending"); return GST_BAD_INPUT; }
+    }
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+    (void)hipGetLastError();
+    set_err("no HIP device");
+    return GST_NO_DEVICE;
+  }
+  auto c = std::make_unique<gst_ctx>();
+  if (opts) c->opts = *opts;
+  c->dev = c->opts.device;
+  if (c->dev < 0 || c->dev >= ndev) { set_err("bad device ordinal"); return GST_BAD_INPUT; }
+  HIPCHK(hipSetDevice(c->dev));
+  HIPCHK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+  c->n = n;
+  c->m = m;
+  c->n_groups = groups->n_groups;
+  c->split = c->opts.degree_split > 0 ? c->opts.degree_split : 32;
+  c->chunk = c->opts.chunk_edges > 0 ? ((c->opts.chunk_edges + 63) / 64) * 64 : 512;
+  c->max_batch = c->opts.max_batch > 0 ? c->opts.max_batch : 64;
+  std::vector<int32_t> beg32(n + 1);
+  c->h_deg.resize(n);
+  for (int32_t v = 0; v <= n; v++) beg32[v] = (int32_t)csr->beg_pos[v];
+  for (int32_t v = 0; v < n; v++) c->h_deg[v] = beg32[v + 1] - beg32[v];
+  c->h_goffs.assign(groups->offs, groups->offs + groups->n_groups + 1);
+  if (groups->n_groups == 0) c->h_goffs.assign(1, 0);
+  HIPCHK(hipMalloc((void **)&c->d_beg, sizeof(int32_t) * (n + 1)));
+  HIPCHK(hipMalloc((void **)&c->d_adj, sizeof(int32_t) * std::max<int64_t>(m, 1)));
+  HIPCHK(hipMalloc((void **)&c->d_w, sizeof(int32_t) * std::max<int64_t>(m, 1)));
+  HIPCHK(hipMalloc((void **)&c->d_goffs, sizeof(long long) * c->h_goffs.size()));
+  HIPCHK(hipMalloc((void **)&c->d_gverts, sizeof(int32_t) * std::max<int64_t>(total, 1)));
+  HIPCHK(hipMemcpy(c->d_beg, beg32.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice));
+  if (m) {
+    HIPCHK(hipMemcpy(c->d_adj, csr->adj, sizeof(int32_t) * m, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_w, csr->w, sizeof(int32_t) * m, hipMemcpyHostToDevice));
+  }
+  {
+    std::vector<long long> go(c->h_goffs.begin(), c->h_goffs.end());
+    HIPCHK(hipMemcpy(c->d_goffs, go.data(), sizeof(long long) * go.size(), hipMemcpyHostToDevice));
+  }
+  if (total) HIPCHK(hipMemcpy(c->d_gverts, groups->verts, sizeof(int32_t) * total, hipMemcpyHostToDevice));
+  c->graph_bytes = (int64_t)4 * (n + 1) + 8 * m + 8 * (int64_t)c->h_goffs.size() + 4 * total;
+  HIPCHK(hipMalloc((void **)&c->d_jc, sizeof(JobConst) * c->max_batch));
+  HIPCHK(hipMalloc((void **)&c->d_js, sizeof(JobState) * c->max_batch));
+  HIPCHK(hipHostMalloc((void **)&c->h_js, sizeof(JobState) * c->max_batch, hipHostMallocDefault));
+  HIPCHK(hipMalloc((void **)&c->d_count, sizeof(unsigned long long)));
+  c->g.beg = c->d_beg; c->g.adj = c->d_adj; c->g.w = c->d_w;
+  c->g.goffs = c->d_goffs; c->g.gverts = c->d_gverts; c->g.n = n;
+  int rc = gst_set_tuning(c.get(), c->split, c->chunk);
+  if (rc) return rc;
+  rc = batch_open(c.get());
+  if (rc) return rc;
+  *out = c.release();
+  return GST_OK;
+}
+
+int gst_set_tuning(gst_ctx *c, int32_t degree_split, int32_t chunk_edges) {
+  if (!c || degree_split < 1 || chunk_edges < 64) { set_err("bad tuning"); return GST_BAD_INPUT; }
+  chunk_edges = ((chunk_edges + 63) / 64) * 64;
+  long long bcap = 0;
+  for (int32_t v = 0; v < c->n; v++)
+    if (c->h_deg[v] >= degree_split) bcap += (c->h_deg[v] + chunk_edges - 1) / chunk_edges;
+  bcap = std::max<long long>(bcap, 1);
+  if (bcap > c->bcap) {
+    // slots sized for the old capacity must be rebuilt
+    slot_free(c->plain);
+    batch_release_slots(c);
+  }
+  c->bcap = std::max(c->bcap, bcap);
+  c->split = degree_split;
+  c->chunk = chunk_edges;
+  c->g.split = degree_split;
+  c->g.chunk = chunk_edges;
+  return GST_OK;
+}
+
+int gst_solve(gst_ctx *c, const int32_t *gids, int32_t k, int32_t D, int32_t flags, gst_result *out) {
+  if (!c || !out) { set_err("null argument"); return GST_BAD_INPUT; }
+  std::memset(out, 0, sizeof(*out));
+  std::lock_guard<std::mutex> lk(c->solve_mu);
+  const auto t0 = clk::now();
+  Query q;
+  int rc = check_query(c, gids, k, D, flags, q);
+  if (rc) { out->status = rc; out->cost = -1; return rc; }
+  HIPCHK(hipSetDevice(c->dev));
+  Slot &s = c->plain;
+  if (!s.T || s.K != q.K || s.D != q.D) {
+    // reuse a larger-or-equal slot only when the shape matches exactly; else reallocate
+    rc = slot_alloc(c, s, q.K, q.D);
+    if (rc) { out->status = rc; out->cost = -1; return rc; }
+  }
+  JobConst jc;
+  fill_jobconst(c, s, q, jc);
+  JobState js;
+  init_jobstate(js);
+  HIPCHK(hipMemcpyAsync(c->d_jc, &jc, sizeof(jc), hipMemcpyHostToDevice, c->st));
+  HIPCHK(hipMemcpyAsync(c->d_js, &js, sizeof(js), hipMemcpyHostToDevice, c->st));
+  launch_job_init(c->st, s, c);
+  HIPCHK(hipGetLastError());
+  const GraphDev g = c->g;
+  JobState &h = c->h_js[0];
+  double t_dp = 0;
+  int guard = 0;
+  const unsigned merge_grid = 2048;
+  bool dp_round = false;  // previous round was part of the DP loop
+  auto tr0 = clk::now();
+  for (;;) {
+    k_step<<<1, 64, 0, c->st>>>(c->d_jc, c->d_js, 1);
+    HIPCHK(hipMemcpyAsync(&h, c->d_js, sizeof(JobState), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    if (dp_round) t_dp += secs(tr0, clk::now());
+    tr0 = clk::now();
+    if (h.phase == PH_DONE) break;
+    if (++guard > (1 << 22)) { set_err("round limit exceeded"); out->status = GST_INTERNAL; return GST_INTERNAL; }
+    const int ph = h.phase;
+    const dim3 one(1, 1);
+    switch (ph) {
+      case PH_SEED: {
+        long long tot = 0;
+        for (int i = 0; i < q.K; i++) tot += c->h_goffs[q.gids[i] + 1] - c->h_goffs[q.gids[i]];
+        k_seed<<<dim3(grid_for(tot, BLK, 1024), 1), BLK, 0, c->st>>>(c->d_jc, c->d_js, g);
+        break;
+      }
+      case PH_GROW: {
+        const int li = q.D >= 0 ? h.depth : h.parity;
+        const int n0 = h.cnt[li][0], n1 = h.cnt[li][1];
+        if (n0) k_grow_thread<<<dim3(grid_for(n0, BLK, 1 << 20), 1), BLK, 0, c->st>>>(c->d_jc, c->d_js, g);
+        if (n1) k_grow_wave<<<dim3(grid_for(n1, BLK / WAVE, 1 << 20), 1), BLK, 0, c->st>>>(c->d_jc, c->d_js, g);
+        k_clear<<<dim3(grid_for(n0 + n1, BLK, 1 << 16), 1), BLK, 0, c->st>>>(c->d_jc, c->d_js, g);
+        break;
+      }
+      case PH_HOPCLEAR: {
+        const int n0 = h.cnt[q.D][0], n1 = h.cnt[q.D][1];
+        k_clear<<<dim3(grid_for(n0 + n1, BLK, 1 << 16), 1), BLK, 0, c->st>>>(c->d_jc, c->d_js, g);
+        break;
+      }
+      case PH_TABLES:
+        k_tables<<<one, BLK, 0, c->st>>>(c->d_jc, c->d_js, g);
+        break;
+      case PH_MERGE:
+        if (q.D >= 0) k_merge_hop<<<dim3(merge_grid, 1), BLK, 0, c->st>>>(c->d_jc, c->d_js, g);
+        else k_merge<<<dim3(merge_grid, 1), BLK, 0, c->st>>>(c->d_jc, c->d_js, g);
+        break;
+      case PH_FINAL:
+        k_final<<<dim3(grid_for((long long)c->n * jc.H, BLK, 4096), 1), BLK, 0, c->st>>>(c->d_jc, c->d_js, g);
+        break;
+      case PH_REBUILD:
+        k_rebuild<<<one, BLK, 0, c->st>>>(c->d_jc, c->d_js, g);
+        break;
+      default:
+        break;
+    }
+    HIPCHK(hipGetLastError());
+    // DP-loop time: merge and grow rounds after the level-1 SSSP (hop: every round)
+    dp_round = (ph == PH_MERGE || ph == PH_GROW || ph == PH_HOPCLEAR) && (q.D >= 0 || h.level >= 2);
+  }
+  std::vector<int32_t> edges;
+  if (h.n_out > 0 && h.ans != ~0ull) {
+    edges.resize((size_t)3 * std::min(h.n_out, ECAP));
+    HIPCHK(hipMemcpy(edges.data(), s.out, sizeof(int32_t) * edges.size(), hipMemcpyDeviceToHost));
+  }
+  JobState fin = h;
+  finish_result(c, q, fin, edges, out);
+  if (q.flags & GST_F_STATS) {
+    HIPCHK(hipMemsetAsync(c->d_count, 0, sizeof(unsigned long long), c->st));
+    k_count_finite<<<grid_for((long long)s.arena_words, BLK, 4096), BLK, 0, c->st>>>(s.T, (long long)s.arena_words, c->d_count);
+    unsigned long long cnt = 0;
+    HIPCHK(hipMemcpyAsync(&cnt, c->d_count, sizeof(cnt), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    out->states_finite = (int64_t)cnt;
+  }
+  out->t_dp_s = t_dp;
+  out->t_total_s = secs(t0, clk::now());
+  return out->status < 0 ? out->status : GST_OK;
+}
+
+void gst_result_free(gst_result *r) {
+  if (r && r->edges) { std::free(r->edges); r->edges = nullptr; }
+}
+
+int gst_info_get(gst_ctx *c, gst_info *info) {
+  if (!c || !info) return GST_BAD_INPUT;
+  std::memset(info, 0, sizeof(*info));
+  info->n = c->n;
+  info->m = c->m;
+  info->graph_bytes = c->graph_bytes;
+  batch_info(c, info);
+  return GST_OK;
+}
+
+void gst_close(gst_ctx *c) {
+  if (!c) return;
+  (void)hipSetDevice(c->dev);
+  batch_close(c);
+  slot_free(c->plain);
+  (void)hipFree(c->d_beg); (void)hipFree(c->d_adj); (void)hipFree(c->d_w);
+  (void)hipFree(c->d_goffs); (void)hipFree(c->d_gverts);
+  (void)hipFree(c->d_jc); (void)hipFree(c->d_js); (void)hipHostFree(c->h_js);
+  (void)hipFree(c->d_count);
+  if (c->st) (void)hipStreamDestroy(c->st);
+  delete c;
+}
+
+}  // extern "C"

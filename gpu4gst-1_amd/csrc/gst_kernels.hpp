@@ -1,0 +1,838 @@
+// gst_kernels.hpp — CDNA4 (gfx950, wave64) device code of the GST DP engine.
+//
+// Every kernel is multi-job: gridDim.y indexes a job (one query), blockIdx.x strides
+// over that job's work. A job advances through phases decided on the device by
+// k_step, so the same kernels serve the single-query path (host reads the job state
+// after every round, as the reference does at code/TrimCDP/src/DPQ.cu:512-544) and the
+// batched path (rounds captured in a HIP graph, no host round trips).
+//
+// DP state T(v,p[,d]) = cost of the cheapest tree rooted at v that covers the query
+// groups in subset p (and has height d, hop variant). Arena layout: int32, vertex-major
+// rows, T[(v*H + d)*W + p] with W = 2^k, H = D+1 (H = 1 without a diameter bound), so a
+// vertex's states for one height sit in W consecutive words.
+//
+// Non-hop schedule (level = popcount of p):
+//   level 1 : k-source SSSP = the singleton states (grow rounds)          [SSSP, a4]
+//   tables  : group distance table and Held-Karp tour tables              [a5]
+//   level L : merge kernel (row staged in LDS), then grow rounds          [a8]
+//   level k : merge only; answer = min_v T(v, full)                       [a9]
+// Pruning (all admissible, so the optimum is preserved):
+//   * upper bound UB = min over realizable trees: T(v,p) + sum_{i not in p} T(v,{i})
+//     (tree plus shortest paths) and T(v,p) + T(v,~p); the reference uses the same
+//     two updates (DPQ.cu:82-107);
+//   * a state is grown only if cost <= UB/2: in an optimal tree rooted at its weighted
+//     centroid every grown subtree weighs at most half the optimum (the reference's
+//     "cost-1 > best/2" cut, DPQ.cu:108, and CPUNONHOP.h:795);
+//   * a state is grown only if cost + lb <= UB, lb = max(one-label, tour, tour-2)
+//     bounds computed from lower bounds of the group distances (DPQ.cu:174-224,
+//     CPUNONHOP.h:436-496).
+//   Level-1 growth uses only the UB/2 rule, so every singleton value <= UB_final/2 is an
+//   exact distance and dl_i(v) = min(T(v,{i}), UB_final/2 + 1) is a valid lower bound.
+// Hop schedule: no pruning (the reference's D-PrunedDP++ and D-TrimCDP prune nothing);
+//   per level the merge kernel fills all heights, then heights 0..D-1 are grown in order
+//   (a grow raises the height by exactly one, so each height is final before it is grown).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gst {
+
+constexpr int32_t INF = 0x3fffffff;   // unreachable; sum of two INF fits int32
+constexpr int32_t LB_CAP = 1 << 24;   // lower bounds are clamped here (still admissible)
+constexpr int BLK = 256;              // threads per block (4 wavefronts)
+constexpr int WAVE = 64;
+constexpr int MAXK = 7;
+constexpr int MAXW = 1 << MAXK;
+constexpr int MAXLISTS = 16;          // non-hop: 2 (parity); hop: D+1 <= 16
+
+enum Phase : int32_t {
+  PH_INIT = 0, PH_SEED = 1, PH_GROW = 2, PH_TABLES = 3, PH_MERGE = 4,
+  PH_FINAL = 5, PH_REBUILD = 6, PH_HOPCLEAR = 7, PH_DONE = 8
+};
+
+enum Stat : int32_t {
+  ST_VERTS = 0,   // frontier vertices processed (|S|)
+  ST_EDGES = 1,   // edges scanned (E_S)
+  ST_STATES = 2,  // frontier states evaluated (|F|)
+  ST_RELAX = 3,   // state-edge relaxations (R)
+  ST_PUSH = 4,    // worklist pushes (|F_next|, vertices)
+  ST_ATOM = 5,    // atomicMin issued on the arena
+  ST_GROWR = 6,   // grow rounds
+  ST_MERGE = 7    // merged states written
+};
+
+struct JobConst {
+  int32_t *T;                 // arena
+  unsigned long long *act;    // [nl][n] active-subset masks (bit j = j-th subset of the level)
+  int32_t *lst;               // [nl][n] thread-per-vertex worklists
+  int32_t *blst;              // [nl][bcap][2] (vertex, chunk) wavefront work items
+  int32_t *tab;               // tour tables: hk[K][K][W], w2[K][W], vv[K][K]
+  int32_t *stk;               // rebuild stack: scap x (v, p, d)
+  int32_t *out;               // rebuilt tree: ecap x (u, v, w)
+  long long bcap;
+  int32_t scap, ecap;
+  int32_t K, D, H, W, R, hop, flags, nl;
+  int32_t gid[8];
+  uint8_t nsub[8];
+  uint8_t sub[8][64];         // sub[L][j]: j-th subset of popcount L, ascending
+  uint8_t rank[128];          // rank of p within its popcount level
+};
+
+struct alignas(16) JobState {
+  int32_t phase, level, depth, parity;
+  int32_t cnt[MAXLISTS][2];   // [list][0 = thread items, 1 = wavefront items]
+  int32_t ub, r1, err, n_out;
+  unsigned long long ans;     // (cost << 32) | (v*H + d), min-reduced
+  int32_t rounds, dp_rounds, dirty, pad1;
+  unsigned long long st[8];
+};
+
+struct GraphDev {
+  const int32_t *beg;   // [n+1]
+  const int32_t *adj;   // [m]
+  const int32_t *w;     // [m]
+  const long long *goffs;
+  const int32_t *gverts;
+  int32_t n;
+  int32_t split;        // degree split thread/wavefront
+  int32_t chunk;        // edges per wavefront item
+  int32_t pad;
+};
+
+constexpr int JF_NO_PRUNE = 4;  // == GST_F_NO_PRUNE
+constexpr int JF_NO_TREE = 2;   // == GST_F_NO_TREE
+
+__device__ __forceinline__ int popc(uint32_t x) { return __builtin_popcount(x); }
+
+__device__ __forceinline__ int load_ub(const JobState *s) {
+  return __hip_atomic_load(&s->ub, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// list index of the worklist a job is reading (cur) / writing (nxt) this round
+__device__ __forceinline__ int cur_list(const JobConst &c, const JobState *s) {
+  return c.hop ? s->depth : s->parity;
+}
+__device__ __forceinline__ int nxt_list(const JobConst &c, const JobState *s) {
+  return c.hop ? s->depth + 1 : (s->parity ^ 1);
+}
+
+// Append vertex u to worklist li: a thread item when deg < split, otherwise one
+// wavefront item per chunk of edges.
+__device__ __forceinline__ void push_vertex(const JobConst &c, JobState *s, const GraphDev &g,
+                                            int li, int32_t u) {
+  const int32_t d = g.beg[u + 1] - g.beg[u];
+  if (d < g.split) {
+    const int pos = atomicAdd(&s->cnt[li][0], 1);
+    c.lst[(size_t)li * g.n + pos] = u;
+  } else {
+    const int nch = (d + g.chunk - 1) / g.chunk;
+    const int pos = atomicAdd(&s->cnt[li][1], nch);
+    int32_t *b = c.blst + ((size_t)li * c.bcap + pos) * 2;
+    for (int i = 0; i < nch; i++) { b[2 * i] = u; b[2 * i + 1] = i; }
+  }
+}
+
+// Admissible lower bound on the weight still needed to connect a tree rooted at v
+// (covering p) to the groups outside p. row points at v's non-hop row (W words).
+__device__ int lb_tail(const int32_t *row, int32_t r1, int p, const JobConst &c,
+                       const int32_t *hk, const int32_t *w2) {
+  const int X = c.R ^ p;
+  if (X == 0) return 0;
+  const int K = c.K, W = c.W;
+  int dl[MAXK];
+#pragma unroll
+  for (int i = 0; i < MAXK; i++) {
+    int x = (i < K) ? row[1 << i] : INF;
+    x = x < r1 ? x : r1;
+    dl[i] = x < LB_CAP ? x : LB_CAP;
+  }
+  int one = 0, mind = INF, lb1 = INF, lb2 = 0;
+#pragma unroll
+  for (int a = 0; a < MAXK; a++) {
+    if (a < K && ((X >> a) & 1)) {
+      one = dl[a] > one ? dl[a] : one;
+      mind = dl[a] < mind ? dl[a] : mind;
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < MAXK; a++) {
+    if (!(a < K && ((X >> a) & 1))) continue;
+#pragma unroll
+    for (int b = 0; b < MAXK; b++) {
+      if (!(b < K && ((X >> b) & 1))) continue;
+      const int t = (dl[a] + hk[(a * K + b) * W + X] + dl[b]) >> 1;
+      lb1 = t < lb1 ? t : lb1;
+    }
+    const int t2 = (dl[a] + w2[a * W + X] + mind) >> 1;
+    lb2 = t2 > lb2 ? t2 : lb2;
+  }
+  int lb = lb1 > lb2 ? lb1 : lb2;
+  return lb > one ? lb : one;
+}
+
+// Realizable upper bounds from state (v,p) with cost cst: tree + shortest paths to the
+// missing groups, and tree + complementary tree when that level is already final.
+__device__ __forceinline__ int ub_candidate(const int32_t *row, int p, int cst, int L,
+                                            const JobConst &c) {
+  long long star = cst;
+  bool ok = true;
+  for (int i = 0; i < c.K; i++)
+    if (!((p >> i) & 1)) {
+      const int x = row[1 << i];
+      if (x >= INF) { ok = false; break; }
+      star += x;
+    }
+  long long best = ok ? star : (long long)INF;
+  const int q = c.R ^ p;
+  if (q && popc(q) < L) {
+    const int x = row[q];
+    if (x < INF && (long long)cst + x < best) best = (long long)cst + x;
+  }
+  return best < INF ? (int)best : INF;
+}
+
+// --------------------------------------------------------------------------- step
+// One thread per job: turn the result of the previous round into this round's phase.
+__global__ void k_step(JobConst *jcs, JobState *jss, int njobs) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= njobs) return;
+  const JobConst &c = jcs[j];
+  JobState &s = jss[j];
+  int ph = s.phase;
+  if (ph == PH_DONE) return;
+  s.rounds++;
+  switch (ph) {
+    case PH_INIT:
+      s.level = 1; s.depth = 0; s.parity = 0;
+      s.phase = PH_SEED;
+      return;
+    case PH_SEED:
+      if (c.K == 1) { s.phase = PH_FINAL; return; }
+      if (c.hop) { s.depth = 0; goto hop_entry; }
+      goto grow_entry;
+    case PH_GROW:
+      if (c.hop) { s.depth++; goto hop_entry; }
+      goto grow_entry;
+    case PH_TABLES:
+      s.level = 2;
+      s.phase = PH_MERGE;
+      return;
+    case PH_MERGE:
+      if (s.level == c.K) { s.phase = PH_FINAL; return; }
+      if (c.hop) { s.depth = 0; goto hop_entry; }
+      goto grow_entry;
+    case PH_HOPCLEAR:
+      for (int i = 0; i < MAXLISTS; i++) { s.cnt[i][0] = 0; s.cnt[i][1] = 0; }
+      s.level++;
+      s.depth = 0;
+      s.phase = PH_MERGE;
+      return;
+    case PH_FINAL:
+      s.phase = (s.ans != ~0ull && !(c.flags & JF_NO_TREE)) ? PH_REBUILD : PH_DONE;
+      return;
+    case PH_REBUILD:
+      s.phase = PH_DONE;
+      return;
+    default:
+      s.err = 99;
+      s.phase = PH_DONE;
+      return;
+  }
+grow_entry: {
+    const int nx = s.parity ^ 1;
+    if (s.cnt[nx][0] + s.cnt[nx][1] == 0) {  // level converged
+      s.cnt[s.parity][0] = 0; s.cnt[s.parity][1] = 0;
+      if (s.level == 1) { s.phase = PH_TABLES; return; }
+      s.level++;
+      s.phase = PH_MERGE;
+      return;
+    }
+    s.cnt[s.parity][0] = 0; s.cnt[s.parity][1] = 0;  // consumed last round
+    s.parity = nx;
+    s.phase = PH_GROW;
+    s.dp_rounds++;
+    return;
+  }
+hop_entry: {
+    while (s.depth < c.D && s.cnt[s.depth][0] + s.cnt[s.depth][1] == 0) s.depth++;
+    if (s.depth >= c.D) { s.phase = PH_HOPCLEAR; return; }
+    s.phase = PH_GROW;
+    s.dp_rounds++;
+    return;
+  }
+}
+
+// --------------------------------------------------------------------------- fill
+__global__ void k_fill_i32(int32_t *p, long long n, int32_t v) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  const long long n4 = n >> 2;
+  int4 *p4 = reinterpret_cast<int4 *>(p);
+  const int4 v4 = make_int4(v, v, v, v);
+  for (long long k = i; k < n4; k += stride) p4[k] = v4;
+  for (long long k = (n4 << 2) + i; k < n; k += stride) p[k] = v;
+}
+
+__global__ void k_fill_u64(unsigned long long *p, long long n, unsigned long long v) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long k = i; k < n; k += stride) p[k] = v;
+}
+
+// --------------------------------------------------------------------------- seed
+// T(x, {i}) = 0 for every member x of query group i (height 0); activates bit i.
+__global__ void k_seed(const JobConst *jcs, JobState *jss, GraphDev g) {
+  const JobConst &c = jcs[blockIdx.y];
+  JobState *s = &jss[blockIdx.y];
+  if (s->phase != PH_SEED) return;
+  const int li = c.hop ? 0 : 1;
+  const int stride = gridDim.x * blockDim.x;
+  for (int i = 0; i < c.K; i++) {
+    const long long b = g.goffs[c.gid[i]], e = g.goffs[c.gid[i] + 1];
+    for (long long t = b + blockIdx.x * blockDim.x + threadIdx.x; t < e; t += stride) {
+      const int32_t x = g.gverts[t];
+      c.T[(size_t)x * c.H * c.W + (1 << i)] = 0;
+      const unsigned long long old = atomicOr(&c.act[(size_t)li * g.n + x], 1ull << i);
+      if (old == 0) push_vertex(c, s, g, li, x);
+    }
+  }
+}
+
+// --------------------------------------------------------------------------- grow
+// Evaluate the active states of v (bits of a) at the current level: update UB, and
+// return the subset of bits that may grow. Non-hop only.
+__device__ __forceinline__ unsigned long long grow_select(const JobConst &c, JobState *s,
+                                                          const int32_t *row, unsigned long long a,
+                                                          int L, int ub, int &n_states) {
+  const int32_t *hk = c.tab;
+  const int32_t *w2 = c.tab + c.K * c.K * c.W;
+  const bool prune = !(c.flags & JF_NO_PRUNE);
+  unsigned long long gm = 0;
+  int ubl = ub;
+  while (a) {
+    const int j = __builtin_ctzll(a);
+    a &= a - 1;
+    n_states++;
+    const int p = c.sub[L][j];
+    const int cst = row[p];
+    if (cst >= INF) continue;
+    const int cand = ub_candidate(row, p, cst, L, c);
+    if (cand < ubl) { ubl = cand; atomicMin(&s->ub, cand); }
+    if (prune) {
+      if (cst > (ubl >> 1)) continue;
+      if (L > 1 && cst + lb_tail(row, s->r1, p, c, hk, w2) > ubl) continue;
+    }
+    gm |= 1ull << j;
+  }
+  return gm;
+}
+
+// Relax edge (v -> u, wt) for every bit of gm: atomicMin on u's states; returns the
+// bits that improved.
+__device__ __forceinline__ unsigned long long relax_edge(int32_t *Tsrc, int32_t *Tdst,
+                                                         const JobConst &c, int L,
+                                                         unsigned long long gm, int32_t wt,
+                                                         int &n_atom) {
+  unsigned long long imp = 0;
+  while (gm) {
+    const int j = __builtin_ctzll(gm);
+    gm &= gm - 1;
+    const int p = c.sub[L][j];
+    int nc = Tsrc[p] + wt;
+    nc = nc < INF ? nc : INF;
+    if (nc < Tdst[p]) {
+      n_atom++;
+      const int old = atomicMin(&Tdst[p], nc);
+      if (nc < old) imp |= 1ull << j;
+    }
+  }
+  return imp;
+}
+
+struct GrowCounters {
+  int verts, edges, states, relax, push, atom;
+  __device__ void zero() { verts = edges = states = relax = push = atom = 0; }
+};
+
+__device__ __forceinline__ void flush_counters(JobState *s, GrowCounters &k) {
+  // wave reduction then one atomic per wavefront
+  unsigned long long v[6] = {(unsigned long long)k.verts, (unsigned long long)k.edges,
+                             (unsigned long long)k.states, (unsigned long long)k.relax,
+                             (unsigned long long)k.push, (unsigned long long)k.atom};
+#pragma unroll
+  for (int q = 0; q < 6; q++) {
+    unsigned long long x = v[q];
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    v[q] = x;
+  }
+  if ((threadIdx.x & 63) == 0) {
+    const int map[6] = {ST_VERTS, ST_EDGES, ST_STATES, ST_RELAX, ST_PUSH, ST_ATOM};
+#pragma unroll
+    for (int q = 0; q < 6; q++)
+      if (v[q]) atomicAdd(&s->st[map[q]], v[q]);
+  }
+}
+
+// Thread per vertex (degree < split).
+__global__ __launch_bounds__(BLK) void k_grow_thread(const JobConst *jcs, JobState *jss, GraphDev g) {
+  const JobConst &c = jcs[blockIdx.y];
+  JobState *s = &jss[blockIdx.y];
+  if (s->phase != PH_GROW) return;
+  const int li = cur_list(c, s), lo = nxt_list(c, s);
+  const int cnt = s->cnt[li][0];
+  const int L = s->level;
+  const int RW = c.H * c.W;
+  const int ub = load_ub(s);
+  GrowCounters k;
+  k.zero();
+  for (int i = blockIdx.x * BLK + threadIdx.x; i < cnt; i += gridDim.x * BLK) {
+    const int32_t v = c.lst[(size_t)li * g.n + i];
+    const unsigned long long a = c.act[(size_t)li * g.n + v];
+    k.verts++;
+    int32_t *src = c.T + (size_t)v * RW + (c.hop ? s->depth * c.W : 0);
+    unsigned long long gm;
+    if (c.hop) { gm = a; k.states += popc((uint32_t)a) + popc((uint32_t)(a >> 32)); }
+    else gm = grow_select(c, s, src, a, L, ub, k.states);
+    if (!gm) continue;
+    const int nb = popc((uint32_t)gm) + popc((uint32_t)(gm >> 32));
+    const int32_t e0 = g.beg[v], e1 = g.beg[v + 1];
+    k.edges += e1 - e0;
+    k.relax += (e1 - e0) * nb;
+    for (int32_t e = e0; e < e1; e++) {
+      const int32_t u = g.adj[e];
+      int32_t *dst = c.T + (size_t)u * RW + (c.hop ? (s->depth + 1) * c.W : 0);
+      const unsigned long long imp = relax_edge(src, dst, c, L, gm, g.w[e], k.atom);
+      if (imp) {
+        const unsigned long long old = atomicOr(&c.act[(size_t)lo * g.n + u], imp);
+        if (old == 0) { push_vertex(c, s, g, lo, u); k.push++; }
+      }
+    }
+  }
+  flush_counters(s, k);
+}
+
+// Wavefront per (vertex, edge chunk) for vertices with degree >= split.
+__global__ __launch_bounds__(BLK) void k_grow_wave(const JobConst *jcs, JobState *jss, GraphDev g) {
+  const JobConst &c = jcs[blockIdx.y];
+  JobState *s = &jss[blockIdx.y];
+  if (s->phase != PH_GROW) return;
+  const int li = cur_list(c, s), lo = nxt_list(c, s);
+  const int cnt = s->cnt[li][1];
+  const int L = s->level;
+  const int RW = c.H * c.W;
+  const int lane = threadIdx.x & 63;
+  const int ub = load_ub(s);
+  GrowCounters k;
+  k.zero();
+  const int waves_per_grid = gridDim.x * (BLK / WAVE);
+  for (int it = blockIdx.x * (BLK / WAVE) + (threadIdx.x >> 6); it < cnt; it += waves_per_grid) {
+    const int32_t *item = c.blst + ((size_t)li * c.bcap + it) * 2;
+    const int32_t v = item[0], ch = item[1];
+    const unsigned long long a = c.act[(size_t)li * g.n + v];
+    int32_t *src = c.T + (size_t)v * RW + (c.hop ? s->depth * c.W : 0);
+    unsigned long long gm;
+    int nst = 0;
+    if (c.hop) gm = a;
+    else gm = grow_select(c, s, src, a, L, ub, nst);  // every lane computes the same mask
+    if (lane == 0) {
+      k.verts += ch == 0;
+      k.states += ch == 0 ? (c.hop ? popc((uint32_t)a) + popc((uint32_t)(a >> 32)) : nst) : 0;
+    }
+    if (!gm) continue;
+    const int nb = popc((uint32_t)gm) + popc((uint32_t)(gm >> 32));
+    const int32_t eb = g.beg[v] + ch * g.chunk;
+    const int32_t ee = min(g.beg[v + 1], eb + g.chunk);
+    for (int32_t e = eb + lane; e < ee; e += WAVE) {
+      const int32_t u = g.adj[e];
+      k.edges++;
+      k.relax += nb;
+      int32_t *dst = c.T + (size_t)u * RW + (c.hop ? (s->depth + 1) * c.W : 0);
+      const unsigned long long imp = relax_edge(src, dst, c, L, gm, g.w[e], k.atom);
+      if (imp) {
+        const unsigned long long old = atomicOr(&c.act[(size_t)lo * g.n + u], imp);
+        if (old == 0) { push_vertex(c, s, g, lo, u); k.push++; }
+      }
+    }
+  }
+  flush_counters(s, k);
+}
+
+// Zero the active masks of the list just consumed (after both grow kernels ran).
+__global__ void k_clear(const JobConst *jcs, JobState *jss, GraphDev g) {
+  const JobConst &c = jcs[blockIdx.y];
+  JobState *s = &jss[blockIdx.y];
+  int li;
+  if (s->phase == PH_GROW) li = cur_list(c, s);
+  else if (s->phase == PH_HOPCLEAR) li = c.D;
+  else return;
+  const int n0 = s->cnt[li][0], n1 = s->cnt[li][1];
+  for (int i = blockIdx.x * BLK + threadIdx.x; i < n0 + n1; i += gridDim.x * BLK) {
+    const int32_t v = i < n0 ? c.lst[(size_t)li * g.n + i] : c.blst[((size_t)li * c.bcap + (i - n0)) * 2];
+    c.act[(size_t)li * g.n + v] = 0;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0 && s->phase == PH_GROW) atomicAdd(&s->st[ST_GROWR], 1ull);
+}
+
+// --------------------------------------------------------------------------- tables
+// After level 1: r1 = UB/2 + 1, group distance lower bounds vv[a][b] and Held-Karp tour
+// tables hk[a][b][X] (cheapest walk from group a through every group of X ending at
+// b in X) and w2[a][X] = min_b hk[a][b][X]. One block per job.
+__global__ __launch_bounds__(BLK) void k_tables(const JobConst *jcs, JobState *jss, GraphDev g) {
+  const JobConst &c = jcs[blockIdx.y];
+  JobState *s = &jss[blockIdx.y];
+  if (s->phase != PH_TABLES || blockIdx.x != 0) return;
+  __shared__ int vv[MAXK * MAXK];
+  __shared__ int hk[MAXK * MAXK * MAXW];
+  __shared__ int r1s;
+  const int K = c.K, W = c.W;
+  if (threadIdx.x == 0) {
+    const int ub = load_ub(s);
+    long long r = (long long)(ub >> 1) + 1;
+    r1s = (int)(r < LB_CAP ? r : LB_CAP);
+    s->r1 = r1s;
+  }
+  for (int i = threadIdx.x; i < MAXK * MAXK; i += BLK) vv[i] = LB_CAP;
+  __syncthreads();
+  const int r1 = r1s;
+  for (int a = 0; a < K; a++) {
+    const long long b0 = g.goffs[c.gid[a]], b1 = g.goffs[c.gid[a] + 1];
+    for (long long t = b0 + threadIdx.x; t < b1; t += BLK) {
+      const int32_t x = g.gverts[t];
+      const int32_t *row = c.T + (size_t)x * W;
+      for (int b = 0; b < K; b++) {
+        int d = row[1 << b];
+        d = d < r1 ? d : r1;
+        atomicMin(&vv[a * MAXK + b], d);
+      }
+    }
+  }
+  __syncthreads();
+  // Held-Karp by subset size: parallel over (a, b) for each X
+  for (int X = 1; X < W; X++) {
+    for (int ab = threadIdx.x; ab < K * K; ab += BLK) {
+      const int a = ab / K, b = ab % K;
+      int val = INF;
+      if ((X >> b) & 1) {
+        const int Y = X ^ (1 << b);
+        if (Y == 0) val = vv[a * MAXK + b];
+        else {
+          for (int m = 0; m < K; m++)
+            if ((Y >> m) & 1) {
+              const int t = hk[(a * K + m) * W + Y] + vv[m * MAXK + b];
+              val = t < val ? t : val;
+            }
+        }
+      }
+      hk[(a * K + b) * W + X] = val < LB_CAP ? val : LB_CAP;
+    }
+    __syncthreads();
+  }
+  int32_t *ghk = c.tab, *gw2 = c.tab + K * K * W, *gvv = gw2 + K * W;
+  for (int i = threadIdx.x; i < K * K * W; i += BLK) ghk[i] = (i % W) ? hk[i] : 0;
+  for (int i = threadIdx.x; i < K * W; i += BLK) {
+    const int a = i / W, X = i % W;
+    int best = LB_CAP;
+    for (int b = 0; b < K; b++)
+      if ((X >> b) & 1) best = hk[(a * K + b) * W + X] < best ? hk[(a * K + b) * W + X] : best;
+    gw2[i] = X ? best : 0;
+  }
+  for (int i = threadIdx.x; i < K * K; i += BLK) gvv[i] = vv[(i / K) * MAXK + (i % K)];
+}
+
+// --------------------------------------------------------------------------- merge
+// Non-hop merge for level L: T(v,p) = min over splits p = p1 + p2 of T(v,p1) + T(v,p2).
+// A tile of vertices has its rows staged in LDS (coalesced contiguous read), each
+// thread evaluates (vertex, subset) items, new states are activated for growth.
+constexpr int MERGE_LDS_WORDS = 4096;
+__global__ __launch_bounds__(BLK) void k_merge(const JobConst *jcs, JobState *jss, GraphDev g) {
+  const JobConst &c = jcs[blockIdx.y];
+  JobState *s = &jss[blockIdx.y];
+  if (s->phase != PH_MERGE || c.hop) return;
+  __shared__ int32_t rows[MERGE_LDS_WORDS];
+  __shared__ unsigned long long amask[512];
+  __shared__ int ubs;
+  const int W = c.W, K = c.K, L = s->level, R = c.R;
+  const int TV = min(512, MERGE_LDS_WORDS / W);
+  const int nL = c.nsub[L];
+  const int lo = s->parity ^ 1;
+  const bool prune = !(c.flags & JF_NO_PRUNE);
+  const int32_t *hk = c.tab;
+  const int32_t *w2 = c.tab + K * K * W;
+  const int r1 = s->r1;
+  int merged = 0;
+  const long long ntiles = ((long long)g.n + TV - 1) / TV;
+  for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const long long v0 = tile * TV;
+    const int tv = (int)min((long long)TV, (long long)g.n - v0);
+    for (int i = threadIdx.x; i < tv * W; i += BLK) rows[i] = c.T[(size_t)v0 * W + i];
+    for (int i = threadIdx.x; i < tv; i += BLK) amask[i] = 0;
+    if (threadIdx.x == 0) ubs = load_ub(s);
+    __syncthreads();
+    const int ub = ubs;
+    int ubl = ub;
+    for (int it = threadIdx.x; it < tv * nL; it += BLK) {
+      const int vi = it / nL, j = it - vi * nL;
+      const int p = c.sub[L][j];
+      const int32_t *row = rows + vi * W;
+      const int low = p & -p, rest = p ^ low;
+      int m = INF;
+      for (int sm = rest;; sm = (sm - 1) & rest) {
+        const int p1 = low | sm;
+        if (p1 != p) {
+          const int t = row[p1] + row[p ^ p1];
+          m = t < m ? t : m;
+        }
+        if (sm == 0) break;
+      }
+      if (m >= INF) continue;
+      merged++;
+      c.T[(size_t)(v0 + vi) * W + p] = m;
+      const int cand = (L == K) ? m : ub_candidate(row, p, m, L, c);
+      if (cand < ubl) ubl = cand;
+      if (L < K) {
+        bool grow = true;
+        if (prune) grow = m <= (ub >> 1) && m + lb_tail(row, r1, p, c, hk, w2) <= ub;
+        if (grow) atomicOr(&amask[vi], 1ull << j);
+      }
+    }
+    if (ubl < ub) atomicMin(&ubs, ubl);
+    __syncthreads();
+    if (threadIdx.x == 0 && ubs < ub) atomicMin(&s->ub, ubs);
+    for (int vi = threadIdx.x; vi < tv; vi += BLK) {
+      if (amask[vi]) {
+        const int32_t v = (int32_t)(v0 + vi);
+        c.act[(size_t)lo * g.n + v] = amask[vi];
+        push_vertex(c, s, g, lo, v);
+      }
+    }
+    __syncthreads();
+  }
+  if (merged) atomicAdd(&s->st[ST_MERGE], (unsigned long long)merged);
+}
+
+// Hop merge for level L: T(v,p,max(d1,d2)) = min T(v,p1,d1) + T(v,p2,d2), d1 + d2 <= D.
+constexpr int HMERGE_LDS_WORDS = 8192;
+__global__ __launch_bounds__(BLK) void k_merge_hop(const JobConst *jcs, JobState *jss, GraphDev g) {
+  const JobConst &c = jcs[blockIdx.y];
+  JobState *s = &jss[blockIdx.y];
+  if (s->phase != PH_MERGE || !c.hop) return;
+  __shared__ int32_t rows[HMERGE_LDS_WORDS];
+  __shared__ unsigned long long amask[2048];
+  const int W = c.W, H = c.H, D = c.D, K = c.K, L = s->level;
+  const int RW = H * W;
+  const int TV = max(1, min(256, HMERGE_LDS_WORDS / RW));
+  const int nL = c.nsub[L];
+  int merged = 0;
+  const long long ntiles = ((long long)g.n + TV - 1) / TV;
+  for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const long long v0 = tile * TV;
+    const int tv = (int)min((long long)TV, (long long)g.n - v0);
+    for (int i = threadIdx.x; i < tv * RW; i += BLK) rows[i] = c.T[(size_t)v0 * RW + i];
+    for (int i = threadIdx.x; i < tv * H; i += BLK) amask[i] = 0;
+    __syncthreads();
+    for (int it = threadIdx.x; it < tv * nL * H; it += BLK) {
+      const int vi = it / (nL * H), rem = it - vi * nL * H;
+      const int j = rem / H, nd = rem - j * H;
+      const int p = c.sub[L][j];
+      const int32_t *row = rows + vi * RW;
+      const int low = p & -p, rest = p ^ low;
+      const int lim = min(nd, D - nd);
+      int m = INF;
+      if (lim >= 0) {
+        for (int sm = rest;; sm = (sm - 1) & rest) {
+          const int p1 = low | sm, p2 = p ^ p1;
+          if (p1 != p) {
+            const int a1 = row[nd * W + p1], a2 = row[nd * W + p2];
+            for (int d2 = 0; d2 <= lim; d2++) {  // d1 = nd
+              const int t = a1 + row[d2 * W + p2];
+              m = t < m ? t : m;
+            }
+            for (int d1 = 0; d1 <= lim && d1 < nd; d1++) {  // d2 = nd
+              const int t = row[d1 * W + p1] + a2;
+              m = t < m ? t : m;
+            }
+          }
+          if (sm == 0) break;
+        }
+      }
+      if (m >= INF) continue;
+      merged++;
+      c.T[(size_t)(v0 + vi) * RW + nd * W + p] = m;
+      if (L < K && nd < D) atomicOr(&amask[vi * H + nd], 1ull << j);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < tv * H; i += BLK) {
+      if (amask[i]) {
+        const int vi = i / H, nd = i - vi * H;
+        const int32_t v = (int32_t)(v0 + vi);
+        c.act[(size_t)nd * g.n + v] = amask[i];
+        push_vertex(c, s, g, nd, v);
+      }
+    }
+    __syncthreads();
+  }
+  if (merged) atomicAdd(&s->st[ST_MERGE], (unsigned long long)merged);
+}
+
+// --------------------------------------------------------------------------- final
+// ans = min over v, d of (T(v, full, d) << 32 | v*H + d).
+__global__ __launch_bounds__(BLK) void k_final(const JobConst *jcs, JobState *jss, GraphDev g) {
+  const JobConst &c = jcs[blockIdx.y];
+  JobState *s = &jss[blockIdx.y];
+  if (s->phase != PH_FINAL) return;
+  __shared__ unsigned long long bm;
+  if (threadIdx.x == 0) bm = ~0ull;
+  __syncthreads();
+  unsigned long long best = ~0ull;
+  const int H = c.H, W = c.W, R = c.R;
+  for (long long i = (long long)blockIdx.x * BLK + threadIdx.x; i < (long long)g.n * H;
+       i += (long long)gridDim.x * BLK) {
+    const int32_t cst = c.T[(size_t)i * W + R];
+    if (cst < INF) {
+      const unsigned long long key = ((unsigned long long)cst << 32) | (unsigned long long)i;
+      best = key < best ? key : best;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long o = __shfl_xor(best, off, 64);
+    best = o < best ? o : best;
+  }
+  if ((threadIdx.x & 63) == 0 && best != ~0ull) atomicMin(&bm, best);
+  __syncthreads();
+  if (threadIdx.x == 0 && bm != ~0ull) atomicMin(&s->ans, bm);
+}
+
+// --------------------------------------------------------------------------- rebuild
+// Tree reconstruction from the cost arena alone. For a state of cost c a predecessor
+// with cost(pred) (+ w) <= c always exists (costs only decrease after being set), so
+// following any such predecessor terminates (a grow lowers the cost by >= 1, a merge
+// shrinks the subset) and yields a structure of weight <= c; at the optimum it is an
+// optimal tree. Choices are canonical: smallest split, then smallest edge index.
+__device__ __forceinline__ bool in_group(const GraphDev &g, int grp, int32_t v) {
+  long long lo = g.goffs[grp], hi = g.goffs[grp + 1];
+  while (lo < hi) {
+    const long long mid = (lo + hi) >> 1;
+    if (g.gverts[mid] < v) lo = mid + 1; else hi = mid;
+  }
+  return lo < g.goffs[grp + 1] && g.gverts[lo] == v;
+}
+
+__global__ __launch_bounds__(BLK) void k_rebuild(const JobConst *jcs, JobState *jss, GraphDev g) {
+  const JobConst &c = jcs[blockIdx.y];
+  JobState *s = &jss[blockIdx.y];
+  if (s->phase != PH_REBUILD || blockIdx.x != 0) return;
+  __shared__ int top, nout, err, cv, cp, cd, ccost;
+  __shared__ long long found;
+  const int H = c.H, W = c.W, D = c.D;
+  const int RW = H * W;
+  if (threadIdx.x == 0) {
+    const unsigned long long a = s->ans;
+    const long long idx = (long long)(a & 0xffffffffull);
+    c.stk[0] = (int32_t)(idx / H);
+    c.stk[1] = c.R;
+    c.stk[2] = (int32_t)(idx % H);
+    top = 1; nout = 0; err = 0;
+  }
+  __syncthreads();
+  for (int step = 0; step < (1 << 20); step++) {
+    if (top == 0 || err) break;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      top--;
+      cv = c.stk[3 * top]; cp = c.stk[3 * top + 1]; cd = c.stk[3 * top + 2];
+      ccost = c.T[(size_t)cv * RW + cd * W + cp];
+      found = 0x7fffffffffffffffll;
+    }
+    __syncthreads();
+    const int v = cv, p = cp, d = cd, cost = ccost;
+    // seed: singleton at cost 0 (height 0) held by a member of that group
+    if (popc(p) == 1 && cost == 0 && d == 0) {
+      if (threadIdx.x == 0 && !in_group(g, c.gid[__builtin_ctz(p)], v)) err = 3;
+      __syncthreads();
+      continue;
+    }
+    // merge search: split index = (submask sm of rest) * H*H + d1*H + d2
+    const int low = p & -p, rest = p ^ low;
+    const int nsm = 1 << popc(rest);
+    for (int it = threadIdx.x; it < nsm * H * H; it += BLK) {
+      const int k = it / (H * H), dd = it - k * H * H, d1 = dd / H, d2 = dd - d1 * H;
+      // k-th submask of rest (pdep of k into rest)
+      int sm = 0, kk = k;
+      for (int b = 0; b < MAXK; b++)
+        if ((rest >> b) & 1) { if (kk & 1) sm |= 1 << b; kk >>= 1; }
+      const int p1 = low | sm;
+      if (p1 == p) continue;
+      if (c.hop) {
+        if ((d1 > d2 ? d1 : d2) != d || d1 + d2 > D) continue;
+      } else if (d1 | d2) continue;
+      const long long t = (long long)c.T[(size_t)v * RW + d1 * W + p1] + c.T[(size_t)v * RW + d2 * W + (p ^ p1)];
+      if (t <= cost) atomicMin((unsigned long long *)&found, (unsigned long long)it);
+    }
+    __syncthreads();
+    if (found != 0x7fffffffffffffffll) {
+      if (threadIdx.x == 0) {
+        const int it = (int)found;
+        const int k = it / (H * H), dd = it - k * H * H, d1 = dd / H, d2 = dd - d1 * H;
+        int sm = 0, kk = k;
+        for (int b = 0; b < MAXK; b++)
+          if ((rest >> b) & 1) { if (kk & 1) sm |= 1 << b; kk >>= 1; }
+        const int p1 = low | sm;
+        if (top + 2 > c.scap) err = 4;
+        else {
+          c.stk[3 * top] = v; c.stk[3 * top + 1] = p1; c.stk[3 * top + 2] = d1; top++;
+          c.stk[3 * top] = v; c.stk[3 * top + 1] = p ^ p1; c.stk[3 * top + 2] = d2; top++;
+        }
+      }
+      __syncthreads();
+      continue;
+    }
+    // grow search: neighbour u with T(u, p, d-1) + w(u,v) <= cost
+    if (c.hop && d == 0) {
+      if (threadIdx.x == 0) err = 5;
+      __syncthreads();
+      continue;
+    }
+    const int pd = c.hop ? d - 1 : 0;
+    const int32_t e0 = g.beg[v], e1 = g.beg[v + 1];
+    for (int32_t e = e0 + threadIdx.x; e < e1; e += BLK) {
+      const int32_t u = g.adj[e];
+      const long long t = (long long)c.T[(size_t)u * RW + pd * W + p] + g.w[e];
+      if (t <= cost) atomicMin((unsigned long long *)&found, (unsigned long long)(e - e0));
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      if (found == 0x7fffffffffffffffll) err = 6;
+      else {
+        const int32_t e = e0 + (int32_t)found;
+        const int32_t u = g.adj[e];
+        if (nout >= c.ecap || top + 1 > c.scap) err = 7;
+        else {
+          c.out[3 * nout] = u < v ? u : v;
+          c.out[3 * nout + 1] = u < v ? v : u;
+          c.out[3 * nout + 2] = g.w[e];
+          nout++;
+          c.stk[3 * top] = u; c.stk[3 * top + 1] = p; c.stk[3 * top + 2] = pd; top++;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    s->n_out = nout;
+    if (err) s->err = err;
+    else if (top != 0) s->err = 8;
+  }
+}
+
+// --------------------------------------------------------------------------- stats
+__global__ void k_count_finite(const int32_t *T, long long n, unsigned long long *out) {
+  unsigned long long cnt = 0;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    cnt += T[i] < INF;
+  for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off, 64);
+  if ((threadIdx.x & 63) == 0 && cnt) atomicAdd(out, cnt);
+}
+
+}  // namespace gst

@@ -1,0 +1,146 @@
+/* gst.h — C ABI of the MI355X group Steiner tree engine (libgst.so, HIP for gfx950).
+ *
+ * Drop-in for the DPQ solve of the reference's three GPU programs. The reference has no
+ * FFI: its solve is a C++ call per query with a CSR graph, a group table and k group ids
+ * (plus D for the hop variant). Each entry point below names the reference interface it
+ * replaces:
+ *
+ *   gst_open    replaces the per-run set-up of the drivers: read_input_graph/read_Group +
+ *               toCSR + cudaMallocManaged of the CSR
+ *               (code/TrimCDP/include/exp_GPU_nonHop.h:40-94,
+ *                code/TrimCDP/include/graph_v_of_v_idealID/csr_graph.hpp:39-128,
+ *                code/D-TrimCDP-WB/src/GPUHop.cu:96-151). The graph and the group table
+ *               stay resident on the device until gst_close.
+ *   gst_solve   replaces DP_GPU (code/TrimCDP/include/DPQ.cuh:63, called at
+ *               exp_GPU_nonHop.h:122) when D < 0, and DP_gpu
+ *               (code/D-TrimCDP/include/DPQ.cuh:63, called at exp_GPU_Hop.h:103) and the
+ *               per-query block of D-TrimCDP-WB (code/D-TrimCDP-WB/src/GPUHop.cu:172-265)
+ *               when D >= 0. One query, host-driven rounds (one host sync per round, as the
+ *               reference). Returns the optimal weight, the tree and the counters the
+ *               reference writes to its CSV (time, states, processed states).
+ *   gst_solve_batch  the batched persistent form (no reference counterpart: the
+ *               reference solves queries one at a time, exp_GPU_nonHop.h:95-136): many
+ *               queries of mixed k and D share the device, device-side phase control,
+ *               the round loop captured in a HIP graph.
+ *   gst_submit / gst_poll / gst_wait  serving: queries are admitted as batch slots and
+ *               DP memory (budget V*2^k*(D+1)*4 B per query) free up; results are
+ *               identical to gst_solve's.
+ *
+ * Conventions (reference): group ids are 0-based indices into the group table (the .g
+ * line "g<n>" is group n-1, read_for_GSTP.h:131-133); duplicate ids in a query are
+ * dropped (the reference keeps them in an unordered_set). Weights are integers in
+ * [1, 2^20]. Status codes replace exit() and the reference's cost sentinels (TrimCDP
+ * returns 0 and D-TrimCDP 100000 when there is no solution, DPQ.cu:495-500,571-574 and
+ * code/D-TrimCDP/src/DPQ.cu:229-242); drivers apply those sentinels themselves.
+ * Thread safety: one gst_solve/gst_solve_batch at a time per context; gst_submit and
+ * gst_poll may be called from any thread.
+ */
+#ifndef GST_H
+#define GST_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  GST_OK = 0,
+  GST_NO_SOLUTION = 1,  /* some query group cannot be connected to the others */
+  GST_PENDING = 2,      /* gst_poll: not finished yet */
+  GST_BAD_INPUT = -1,   /* malformed graph/groups/query, k > GST_MAX_K, D > GST_MAX_D */
+  GST_OVER_BUDGET = -2, /* the query's DP arena does not fit the memory budget */
+  GST_DEVICE_ERROR = -3,
+  GST_INTERNAL = -4,    /* reconstruction failed (never expected) */
+  GST_NO_DEVICE = -5
+};
+
+#define GST_MAX_K 7
+#define GST_MAX_D 15
+
+/* flags */
+#define GST_F_STATS 1    /* count finite states after the solve (extra arena scan) */
+#define GST_F_NO_TREE 2  /* skip tree reconstruction */
+#define GST_F_NO_PRUNE 4 /* disable upper/lower-bound pruning (exact either way) */
+
+typedef struct {
+  int32_t n;              /* vertices */
+  int64_t m;              /* directed arcs = 2 x undirected edges; must be < 2^31 */
+  const int64_t *beg_pos; /* [n+1] */
+  const int32_t *adj;     /* [m], each adjacency list ascending */
+  const int32_t *w;       /* [m], symmetric, in [1, 2^20] */
+} gst_csr;
+
+typedef struct {
+  int32_t n_groups;
+  const int64_t *offs;  /* [n_groups+1] */
+  const int32_t *verts; /* members of group g: verts[offs[g] .. offs[g+1]) */
+} gst_groups;
+
+typedef struct {
+  int32_t device;          /* HIP device ordinal */
+  int32_t degree_split;    /* vertices with degree >= this are grown by a wavefront per
+                              edge chunk, the rest by a thread per vertex (0 = 32) */
+  int32_t chunk_edges;     /* edges per wavefront work item (0 = 512) */
+  int32_t max_batch;       /* job slots of the batched/served forms (0 = 64) */
+  int64_t dp_budget_bytes; /* DP memory budget of the batched/served forms
+                              (0 = 75% of free device memory at gst_open) */
+  int32_t flags;           /* default flags for every solve */
+  int32_t graph_rounds;    /* rounds captured per HIP graph launch (0 = 16) */
+} gst_opts;
+
+typedef struct {
+  const int32_t *gids; /* k group ids */
+  int32_t k;
+  int32_t D;           /* diameter bound, < 0 = none */
+  int32_t flags;
+} gst_query;
+
+typedef struct {
+  int32_t status;         /* GST_OK / GST_NO_SOLUTION / error */
+  int32_t cost;           /* optimal weight (-1 when status != GST_OK) */
+  int32_t tree_cost;      /* sum of the distinct edge weights of edges[] */
+  int32_t n_edges;
+  int32_t *edges;         /* n_edges (u, v, w) triples with u < v; owned by the result */
+  int32_t root, root_height;
+  int32_t k, D;           /* k after de-duplication */
+  int32_t rounds;         /* device rounds executed */
+  double t_total_s;       /* submission (or call) to result on the host */
+  double t_dp_s;          /* DP loop only: the reference's GPU timing scope (DPQ.cu:511-555) */
+  double t_queue_s;       /* served form: time spent waiting for admission */
+  int64_t states_finite;  /* GST_F_STATS: states with a finite cost (the CSV "counts") */
+  int64_t states_processed; /* frontier states evaluated (the CSV "process_num") */
+  int64_t grow_vertices, grow_edges, grow_relax, grow_pushes; /* grow-kernel work counters */
+  int64_t arena_bytes;    /* DP arena size V*2^k*(D+1)*4 */
+} gst_result;
+
+typedef struct {
+  int32_t n;
+  int64_t m;
+  int64_t graph_bytes;     /* resident CSR + groups */
+  int64_t budget_bytes;    /* DP budget of the batched/served forms */
+  int64_t in_use_bytes;    /* DP memory held by admitted queries */
+  int32_t active_jobs, queued_jobs;
+  int64_t completed;
+} gst_info;
+
+typedef struct gst_ctx gst_ctx; /* opaque; one per graph */
+
+int gst_open(const gst_csr *csr, const gst_groups *groups, const gst_opts *opts, gst_ctx **ctx);
+int gst_solve(gst_ctx *ctx, const int32_t *gids, int32_t k, int32_t D, int32_t flags,
+              gst_result *out);
+int gst_solve_batch(gst_ctx *ctx, const gst_query *queries, int32_t n, gst_result *out);
+int gst_submit(gst_ctx *ctx, const gst_query *queries, int32_t n, uint64_t *tickets);
+int gst_poll(gst_ctx *ctx, uint64_t ticket, gst_result *out);
+int gst_wait(gst_ctx *ctx, uint64_t ticket, gst_result *out);
+int gst_info_get(gst_ctx *ctx, gst_info *info);
+/* Tunables of the grow kernel (degree split, chunk size); used by the autotuner. */
+int gst_set_tuning(gst_ctx *ctx, int32_t degree_split, int32_t chunk_edges);
+void gst_result_free(gst_result *r);
+void gst_close(gst_ctx *ctx);
+const char *gst_strerror(int code);
+const char *gst_last_error(void);
+int64_t gst_arena_bytes(int32_t n, int32_t k, int32_t D);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

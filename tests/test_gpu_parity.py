@@ -1,0 +1,66 @@
+"""GPU parity: the HIP engine against the CPU oracles, bit-exact (integer costs)."""
+import numpy as np
+import pytest
+
+import pyoracle
+from gpu4gst_1_amd import engine
+
+pytestmark = pytest.mark.gpu
+
+
+def _check(eng, graph, groups, inst, q, D=-1, flags=0):
+    r = eng.solve(q, D=D, flags=flags)
+    o = pyoracle.prunedpp(inst, q) if D < 0 else pyoracle.dprunedpp(inst, q, D)
+    assert r["status"] == engine.GST_OK, r
+    assert r["cost"] == o["tree_cost"] == o["dp_cost"], (q, D, r["cost"], o["tree_cost"], o["dp_cost"])
+    assert r["tree_cost"] == r["cost"], (q, D, r["tree_cost"], r["cost"])
+    ok, why = engine.validate_tree(graph, groups, q, r["edges"], r["cost"], D)
+    assert ok, (q, D, why)
+    return r
+
+
+def test_rmat_k3_all_queries(gpu_engine, rmat, rmat_inst):
+    for i, q in enumerate(rmat.queries[3]):
+        print("k3 query", i, q, flush=True)
+        _check(gpu_engine, rmat.graph, rmat.groups, rmat_inst, q)
+
+
+@pytest.mark.parametrize("k", [1, 2, 4, 5, 6, 7])
+def test_rmat_random_k(gpu_engine, rmat, rmat_inst, k):
+    rng = np.random.default_rng(100 + k)
+    for i in range(6 if k < 7 else 3):
+        q = rng.choice(rmat.groups.n_groups, k, replace=False).astype(np.int32)
+        print("k", k, "query", i, q, flush=True)
+        _check(gpu_engine, rmat.graph, rmat.groups, rmat_inst, q)
+
+
+def test_rmat_unpruned_matches_pruned(gpu_engine, rmat, rmat_inst):
+    for q in rmat.queries[3][:5]:
+        a = gpu_engine.solve(q)
+        b = gpu_engine.solve(q, flags=engine.F_NO_PRUNE)
+        assert a["cost"] == b["cost"], (q, a["cost"], b["cost"])
+
+
+@pytest.mark.parametrize("D", [0, 1, 2, 3, 4, 5])
+def test_rmat_hop(gpu_engine, rmat, rmat_inst, D):
+    for i, q in enumerate(rmat.queries[3][:8]):
+        print("hop D", D, "query", i, q, flush=True)
+        r = gpu_engine.solve(q, D=D)
+        o = pyoracle.dprunedpp(rmat_inst, q, D)
+        if o["status"] != 0:
+            assert r["status"] == engine.GST_NO_SOLUTION, (q, D, r)
+            continue
+        assert r["status"] == engine.GST_OK and r["cost"] == o["tree_cost"] == o["dp_cost"], (q, D, r["cost"], o)
+        ok, why = engine.validate_tree(rmat.graph, rmat.groups, q, r["edges"], r["cost"], D)
+        assert ok, (q, D, why)
+
+
+def test_rmat_hop_k5(gpu_engine, rmat, rmat_inst):
+    rng = np.random.default_rng(7)
+    for i in range(4):
+        q = rng.choice(rmat.groups.n_groups, 5, replace=False).astype(np.int32)
+        for D in (3, 4):
+            print("hop k5 D", D, q, flush=True)
+            r = gpu_engine.solve(q, D=D)
+            o = pyoracle.dprunedpp(rmat_inst, q, D)
+            assert r["cost"] == o["tree_cost"], (q, D, r["cost"], o["tree_cost"])
