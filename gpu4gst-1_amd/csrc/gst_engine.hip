@@ -86,6 +86,7 @@ struct gst_ctx {
   JobState *d_js = nullptr;  // [max_batch]
   JobState *h_js = nullptr;  // pinned mirror [max_batch]
   unsigned long long *d_count = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;  // bracket the work kernels of one round
   int max_batch = 64;
   // batched / served (gst_batch.inc)
   struct Batch;
@@ -367,6 +368,8 @@ int gst_open(const gst_csr *csr, const gst_groups *groups, const gst_opts *opts,
   HIPCHK(hipMalloc((void **)&c->d_js, sizeof(JobState) * c->max_batch));
   HIPCHK(hipHostMalloc((void **)&c->h_js, sizeof(JobState) * c->max_batch, hipHostMallocDefault));
   HIPCHK(hipMalloc((void **)&c->d_count, sizeof(unsigned long long)));
+  HIPCHK(hipEventCreate(&c->ev0));
+  HIPCHK(hipEventCreate(&c->ev1));
   c->g.beg = c->d_beg; c->g.adj = c->d_adj; c->g.w = c->d_w;
   c->g.goffs = c->d_goffs; c->g.gverts = c->d_gverts; c->g.n = n;
   int rc = gst_set_tuning(c.get(), c->split, c->chunk);
@@ -426,12 +429,20 @@ int gst_solve(gst_ctx *c, const int32_t *gids, int32_t k, int32_t D, int32_t fla
   int guard = 0;
   const unsigned merge_grid = 2048;
   bool dp_round = false;  // previous round was part of the DP loop
+  int timed_phase = -1;   // phase whose work kernels ev0/ev1 bracket
+  double t_grow = 0, t_merge = 0;
   auto tr0 = clk::now();
   for (;;) {
     k_step<<<1, 64, 0, c->st>>>(c->d_jc, c->d_js, 1);
     HIPCHK(hipMemcpyAsync(&h, c->d_js, sizeof(JobState), hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
     if (dp_round) t_dp += secs(tr0, clk::now());
+    if (timed_phase == PH_GROW || timed_phase == PH_MERGE) {
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+      (timed_phase == PH_GROW ? t_grow : t_merge) += ms * 1e-3;
+    }
+    timed_phase = -1;
     tr0 = clk::now();
     if (h.phase == PH_DONE) break;
     if (++guard > (1 << 22)) { set_err("round limit exceeded"); out->status = GST_INTERNAL; return GST_INTERNAL; }
@@ -447,8 +458,11 @@ int gst_solve(gst_ctx *c, const int32_t *gids, int32_t k, int32_t D, int32_t fla
       case PH_GROW: {
         const int li = q.D >= 0 ? h.depth : h.parity;
         const int n0 = h.cnt[li][0], n1 = h.cnt[li][1];
+        HIPCHK(hipEventRecord(c->ev0, c->st));
         if (n0) k_grow_thread<<<dim3(grid_for(n0, BLK, 1 << 20), 1), BLK, 0, c->st>>>(c->d_jc, c->d_js, g);
         if (n1) k_grow_wave<<<dim3(grid_for(n1, BLK / WAVE, 1 << 20), 1), BLK, 0, c->st>>>(c->d_jc, c->d_js, g);
+        HIPCHK(hipEventRecord(c->ev1, c->st));
+        timed_phase = PH_GROW;
         k_clear<<<dim3(grid_for(n0 + n1, BLK, 1 << 16), 1), BLK, 0, c->st>>>(c->d_jc, c->d_js, g);
         break;
       }
@@ -461,8 +475,11 @@ int gst_solve(gst_ctx *c, const int32_t *gids, int32_t k, int32_t D, int32_t fla
         k_tables<<<one, BLK, 0, c->st>>>(c->d_jc, c->d_js, g);
         break;
       case PH_MERGE:
+        HIPCHK(hipEventRecord(c->ev0, c->st));
         if (q.D >= 0) k_merge_hop<<<dim3(merge_grid, 1), BLK, 0, c->st>>>(c->d_jc, c->d_js, g);
         else k_merge<<<dim3(merge_grid, 1), BLK, 0, c->st>>>(c->d_jc, c->d_js, g);
+        HIPCHK(hipEventRecord(c->ev1, c->st));
+        timed_phase = PH_MERGE;
         break;
       case PH_FINAL:
         k_final<<<dim3(grid_for((long long)c->n * jc.H, BLK, 4096), 1), BLK, 0, c->st>>>(c->d_jc, c->d_js, g);
@@ -493,6 +510,8 @@ int gst_solve(gst_ctx *c, const int32_t *gids, int32_t k, int32_t D, int32_t fla
     out->states_finite = (int64_t)cnt;
   }
   out->t_dp_s = t_dp;
+  out->t_grow_s = t_grow;
+  out->t_merge_s = t_merge;
   out->t_total_s = secs(t0, clk::now());
   return out->status < 0 ? out->status : GST_OK;
 }
@@ -520,6 +539,8 @@ void gst_close(gst_ctx *c) {
   (void)hipFree(c->d_goffs); (void)hipFree(c->d_gverts);
   (void)hipFree(c->d_jc); (void)hipFree(c->d_js); (void)hipHostFree(c->h_js);
   (void)hipFree(c->d_count);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->st) (void)hipStreamDestroy(c->st);
   delete c;
 }

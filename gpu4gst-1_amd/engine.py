@@ -49,7 +49,8 @@ class GstResult(ctypes.Structure):
                 ("n_edges", ctypes.c_int32), ("edges", ctypes.POINTER(ctypes.c_int32)),
                 ("root", ctypes.c_int32), ("root_height", ctypes.c_int32), ("k", ctypes.c_int32),
                 ("D", ctypes.c_int32), ("rounds", ctypes.c_int32), ("t_total_s", ctypes.c_double),
-                ("t_dp_s", ctypes.c_double), ("t_queue_s", ctypes.c_double), ("states_finite", ctypes.c_int64),
+                ("t_dp_s", ctypes.c_double), ("t_queue_s", ctypes.c_double), ("t_grow_s", ctypes.c_double),
+                ("t_merge_s", ctypes.c_double), ("states_finite", ctypes.c_int64),
                 ("states_processed", ctypes.c_int64), ("grow_vertices", ctypes.c_int64),
                 ("grow_edges", ctypes.c_int64), ("grow_relax", ctypes.c_int64), ("grow_pushes", ctypes.c_int64),
                 ("arena_bytes", ctypes.c_int64)]

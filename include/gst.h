@@ -106,6 +106,8 @@ typedef struct {
   double t_total_s;       /* submission (or call) to result on the host */
   double t_dp_s;          /* DP loop only: the reference's GPU timing scope (DPQ.cu:511-555) */
   double t_queue_s;       /* served form: time spent waiting for admission */
+  double t_grow_s;        /* device time of the grow kernels (HIP events, plain path) */
+  double t_merge_s;       /* device time of the merge kernels (HIP events, plain path) */
   int64_t states_finite;  /* GST_F_STATS: states with a finite cost (the CSV "counts") */
   int64_t states_processed; /* frontier states evaluated (the CSV "process_num") */
   int64_t grow_vertices, grow_edges, grow_relax, grow_pushes; /* grow-kernel work counters */

@@ -63,4 +63,10 @@ def test_rmat_hop_k5(gpu_engine, rmat, rmat_inst):
             print("hop k5 D", D, q, flush=True)
             r = gpu_engine.solve(q, D=D)
             o = pyoracle.dprunedpp(rmat_inst, q, D)
+            if o["status"] != 0:
+                assert r["status"] == engine.GST_NO_SOLUTION and o["tree_cost"] == pyoracle.INF, (q, D, r, o)
+                assert gpu_engine.DP_gpu(q, D)[0] == 100000
+                continue
             assert r["cost"] == o["tree_cost"], (q, D, r["cost"], o["tree_cost"])
+            ok, why = engine.validate_tree(rmat.graph, rmat.groups, q, r["edges"], r["cost"], D)
+            assert ok, (q, D, why)
