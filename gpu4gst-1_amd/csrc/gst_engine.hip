@@ -20,6 +20,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <set>
 #include <string>
 #include <thread>
 #include <vector>
@@ -85,6 +86,7 @@ struct gst_ctx {
   JobConst *d_jc = nullptr;  // [max_batch]
   JobState *d_js = nullptr;  // [max_batch]
   JobState *h_js = nullptr;  // pinned mirror [max_batch]
+  Plan *d_plan = nullptr;    // work layout of the single-query path
   unsigned long long *d_count = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;  // bracket the work kernels of one round
   int max_batch = 64;
@@ -189,6 +191,11 @@ void fill_jobconst(const gst_ctx *c, const Slot &s, const Query &q, JobConst &jc
   jc.K = q.K; jc.D = q.D; jc.H = q.D >= 0 ? q.D + 1 : 1; jc.W = 1 << q.K; jc.R = jc.W - 1;
   jc.hop = q.D >= 0; jc.flags = q.flags; jc.nl = jc.hop ? jc.H : 2;
   for (int i = 0; i < q.K; i++) jc.gid[i] = q.gids[i];
+  long long members = 0;
+  for (int i = 0; i < q.K; i++) members += c->h_goffs[q.gids[i] + 1] - c->h_goffs[q.gids[i]];
+  jc.seed_total = (int32_t)members;
+  jc.tv = jc.hop ? std::max(1, std::min(256, HMERGE_LDS_WORDS / (jc.H * jc.W)))
+                 : std::min(512, MERGE_LDS_WORDS / jc.W);
   int cnt[8] = {0};
   for (int p = 0; p < jc.W; p++) {
     const int L = __builtin_popcount(p);
@@ -368,6 +375,7 @@ int gst_open(const gst_csr *csr, const gst_groups *groups, const gst_opts *opts,
   HIPCHK(hipMalloc((void **)&c->d_js, sizeof(JobState) * c->max_batch));
   HIPCHK(hipHostMalloc((void **)&c->h_js, sizeof(JobState) * c->max_batch, hipHostMallocDefault));
   HIPCHK(hipMalloc((void **)&c->d_count, sizeof(unsigned long long)));
+  HIPCHK(hipMalloc((void **)&c->d_plan, sizeof(Plan)));
   HIPCHK(hipEventCreate(&c->ev0));
   HIPCHK(hipEventCreate(&c->ev1));
   c->g.beg = c->d_beg; c->g.adj = c->d_adj; c->g.w = c->d_w;
@@ -433,7 +441,7 @@ int gst_solve(gst_ctx *c, const int32_t *gids, int32_t k, int32_t D, int32_t fla
   double t_grow = 0, t_merge = 0;
   auto tr0 = clk::now();
   for (;;) {
-    k_step<<<1, 64, 0, c->st>>>(c->d_jc, c->d_js, 1);
+    k_step<<<1, BLK, 0, c->st>>>(c->d_jc, c->d_js, c->d_plan, 1, c->n);
     HIPCHK(hipMemcpyAsync(&h, c->d_js, sizeof(JobState), hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
     if (dp_round) t_dp += secs(tr0, clk::now());
@@ -447,45 +455,44 @@ int gst_solve(gst_ctx *c, const int32_t *gids, int32_t k, int32_t D, int32_t fla
     if (h.phase == PH_DONE) break;
     if (++guard > (1 << 22)) { set_err("round limit exceeded"); out->status = GST_INTERNAL; return GST_INTERNAL; }
     const int ph = h.phase;
-    const dim3 one(1, 1);
+    Plan *pl = c->d_plan;
     switch (ph) {
-      case PH_SEED: {
-        long long tot = 0;
-        for (int i = 0; i < q.K; i++) tot += c->h_goffs[q.gids[i] + 1] - c->h_goffs[q.gids[i]];
-        k_seed<<<dim3(grid_for(tot, BLK, 1024), 1), BLK, 0, c->st>>>(c->d_jc, c->d_js, g);
+      case PH_SEED:
+        k_seed<<<grid_for(jc.seed_total, BLK, 1024), BLK, 0, c->st>>>(c->d_jc, c->d_js, pl, g);
         break;
-      }
       case PH_GROW: {
         const int li = q.D >= 0 ? h.depth : h.parity;
         const int n0 = h.cnt[li][0], n1 = h.cnt[li][1];
         HIPCHK(hipEventRecord(c->ev0, c->st));
-        if (n0) k_grow_thread<<<dim3(grid_for(n0, BLK, 1 << 20), 1), BLK, 0, c->st>>>(c->d_jc, c->d_js, g);
-        if (n1) k_grow_wave<<<dim3(grid_for(n1, BLK / WAVE, 1 << 20), 1), BLK, 0, c->st>>>(c->d_jc, c->d_js, g);
+        if (n0) k_grow_thread<<<grid_for(n0, BLK, 1 << 20), BLK, 0, c->st>>>(c->d_jc, c->d_js, pl, g);
+        if (n1) k_grow_wave<<<grid_for(n1, BLK / WAVE, 1 << 20), BLK, 0, c->st>>>(c->d_jc, c->d_js, pl, g);
         HIPCHK(hipEventRecord(c->ev1, c->st));
         timed_phase = PH_GROW;
-        k_clear<<<dim3(grid_for(n0 + n1, BLK, 1 << 16), 1), BLK, 0, c->st>>>(c->d_jc, c->d_js, g);
+        k_clear<<<grid_for(n0 + n1, BLK, 1 << 16), BLK, 0, c->st>>>(c->d_jc, c->d_js, pl, g);
         break;
       }
       case PH_HOPCLEAR: {
         const int n0 = h.cnt[q.D][0], n1 = h.cnt[q.D][1];
-        k_clear<<<dim3(grid_for(n0 + n1, BLK, 1 << 16), 1), BLK, 0, c->st>>>(c->d_jc, c->d_js, g);
+        k_clear<<<grid_for(n0 + n1, BLK, 1 << 16), BLK, 0, c->st>>>(c->d_jc, c->d_js, pl, g);
         break;
       }
       case PH_TABLES:
-        k_tables<<<one, BLK, 0, c->st>>>(c->d_jc, c->d_js, g);
+        k_tables<<<1, BLK, 0, c->st>>>(c->d_jc, c->d_js, 1, g);
         break;
-      case PH_MERGE:
+      case PH_MERGE: {
+        const long long tiles = ((long long)c->n + jc.tv - 1) / jc.tv;
         HIPCHK(hipEventRecord(c->ev0, c->st));
-        if (q.D >= 0) k_merge_hop<<<dim3(merge_grid, 1), BLK, 0, c->st>>>(c->d_jc, c->d_js, g);
-        else k_merge<<<dim3(merge_grid, 1), BLK, 0, c->st>>>(c->d_jc, c->d_js, g);
+        if (q.D >= 0) k_merge_hop<<<grid_for(tiles, 1, merge_grid), BLK, 0, c->st>>>(c->d_jc, c->d_js, pl, g);
+        else k_merge<<<grid_for(tiles, 1, merge_grid), BLK, 0, c->st>>>(c->d_jc, c->d_js, pl, g);
         HIPCHK(hipEventRecord(c->ev1, c->st));
         timed_phase = PH_MERGE;
         break;
+      }
       case PH_FINAL:
-        k_final<<<dim3(grid_for((long long)c->n * jc.H, BLK, 4096), 1), BLK, 0, c->st>>>(c->d_jc, c->d_js, g);
+        k_final<<<grid_for(((long long)c->n * jc.H + FINAL_CHUNK - 1) / FINAL_CHUNK, 1, 4096), BLK, 0, c->st>>>(c->d_jc, c->d_js, pl, g);
         break;
       case PH_REBUILD:
-        k_rebuild<<<one, BLK, 0, c->st>>>(c->d_jc, c->d_js, g);
+        k_rebuild<<<1, BLK, 0, c->st>>>(c->d_jc, c->d_js, 1, g);
         break;
       default:
         break;
@@ -539,6 +546,7 @@ void gst_close(gst_ctx *c) {
   (void)hipFree(c->d_goffs); (void)hipFree(c->d_gverts);
   (void)hipFree(c->d_jc); (void)hipFree(c->d_js); (void)hipHostFree(c->h_js);
   (void)hipFree(c->d_count);
+  (void)hipFree(c->d_plan);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->st) (void)hipStreamDestroy(c->st);

@@ -1,10 +1,12 @@
 // gst_kernels.hpp — CDNA4 (gfx950, wave64) device code of the GST DP engine.
 //
-// Every kernel is multi-job: gridDim.y indexes a job (one query), blockIdx.x strides
-// over that job's work. A job advances through phases decided on the device by
-// k_step, so the same kernels serve the single-query path (host reads the job state
-// after every round, as the reference does at code/TrimCDP/src/DPQ.cu:512-544) and the
-// batched path (rounds captured in a HIP graph, no host round trips).
+// Every kernel is multi-job: k_step advances each job (one query) to its phase for the
+// round and lays out the round's work items flattened over jobs (Plan: per-kind prefix
+// sums), so one launch with a fixed grid covers the work of every job and a block finds
+// an item's job by binary search. The same kernels serve the single-query path (host
+// reads the job state after every round, as the reference does at
+// code/TrimCDP/src/DPQ.cu:512-544) and the batched path (rounds captured in a HIP graph,
+// no host round trips).
 //
 // DP state T(v,p[,d]) = cost of the cheapest tree rooted at v that covers the query
 // groups in subset p (and has height d, hop variant). Arena layout: int32, vertex-major
@@ -44,6 +46,8 @@ constexpr int WAVE = 64;
 constexpr int MAXK = 7;
 constexpr int MAXW = 1 << MAXK;
 constexpr int MAXLISTS = 16;          // non-hop: 2 (parity); hop: D+1 <= 16
+constexpr int MAXJOBS = 256;          // jobs one launch can carry
+constexpr int FINAL_CHUNK = 1024;     // arena rows per k_final work item
 
 enum Phase : int32_t {
   PH_INIT = 0, PH_SEED = 1, PH_GROW = 2, PH_TABLES = 3, PH_MERGE = 4,
@@ -72,6 +76,8 @@ struct JobConst {
   long long bcap;
   int32_t scap, ecap;
   int32_t K, D, H, W, R, hop, flags, nl;
+  int32_t seed_total;         // members of the query groups (seed work items)
+  int32_t tv;                 // vertices per merge tile
   int32_t gid[8];
   uint8_t nsub[8];
   uint8_t sub[8][64];         // sub[L][j]: j-th subset of popcount L, ascending
@@ -85,6 +91,13 @@ struct alignas(16) JobState {
   unsigned long long ans;     // (cost << 32) | (v*H + d), min-reduced
   int32_t rounds, dp_rounds, dirty, pad1;
   unsigned long long st[8];
+};
+
+// Work items of one round, flattened over jobs: off[kind][j] = first item of job j.
+enum WorkKind : int32_t { WK_SEED = 0, WK_THR = 1, WK_WAVE = 2, WK_CLEAR = 3, WK_MERGE = 4, WK_FINAL = 5, NWK = 6 };
+struct Plan {
+  int32_t njobs, pad;
+  long long off[NWK][MAXJOBS + 1];
 };
 
 struct GraphDev {
@@ -192,12 +205,8 @@ __device__ __forceinline__ int ub_candidate(const int32_t *row, int p, int cst, 
 }
 
 // --------------------------------------------------------------------------- step
-// One thread per job: turn the result of the previous round into this round's phase.
-__global__ void k_step(JobConst *jcs, JobState *jss, int njobs) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= njobs) return;
-  const JobConst &c = jcs[j];
-  JobState &s = jss[j];
+// Turn the result of the previous round into this job's phase for this round.
+__device__ void step_job(const JobConst &c, JobState &s) {
   int ph = s.phase;
   if (ph == PH_DONE) return;
   s.rounds++;
@@ -262,6 +271,57 @@ hop_entry: {
   }
 }
 
+// One block (BLK >= njobs threads): advance every job, then lay out this round's work
+// items per kind as exclusive prefix sums over jobs.
+__global__ __launch_bounds__(BLK) void k_step(const JobConst *jcs, JobState *jss, Plan *pl,
+                                              int njobs, int n) {
+  __shared__ long long cnt[NWK][MAXJOBS];
+  const int j = threadIdx.x;
+  if (j < njobs) {
+    const JobConst &c = jcs[j];
+    JobState &s = jss[j];
+    step_job(c, s);
+    const int ph = s.phase;
+    long long k[NWK] = {0, 0, 0, 0, 0, 0};
+    if (ph == PH_SEED) k[WK_SEED] = c.seed_total;
+    if (ph == PH_GROW) {
+      const int li = c.hop ? s.depth : s.parity;
+      k[WK_THR] = s.cnt[li][0];
+      k[WK_WAVE] = s.cnt[li][1];
+      k[WK_CLEAR] = k[WK_THR] + k[WK_WAVE];
+    }
+    if (ph == PH_HOPCLEAR) k[WK_CLEAR] = s.cnt[c.D][0] + s.cnt[c.D][1];
+    if (ph == PH_MERGE) k[WK_MERGE] = ((long long)n + c.tv - 1) / c.tv;
+    if (ph == PH_FINAL) k[WK_FINAL] = ((long long)n * c.H + FINAL_CHUNK - 1) / FINAL_CHUNK;
+#pragma unroll
+    for (int q = 0; q < NWK; q++) cnt[q][j] = k[q];
+  }
+  __syncthreads();
+  if (j < NWK) {
+    long long acc = 0;
+    pl->off[j][0] = 0;
+    for (int t = 0; t < njobs; t++) { acc += cnt[j][t]; pl->off[j][t + 1] = acc; }
+  }
+  if (j == 0) pl->njobs = njobs;
+}
+
+// Copy the offsets of one work kind into LDS; returns the number of jobs.
+__device__ __forceinline__ int load_plan(const Plan *pl, int kind, long long *soff) {
+  const int nj = pl->njobs;
+  for (int t = threadIdx.x; t <= nj; t += blockDim.x) soff[t] = pl->off[kind][t];
+  __syncthreads();
+  return nj;
+}
+// Job owning flattened item i: the largest j with soff[j] <= i.
+__device__ __forceinline__ int find_job(const long long *soff, int nj, long long i) {
+  int lo = 0, hi = nj - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (soff[mid] <= i) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
 // --------------------------------------------------------------------------- fill
 __global__ void k_fill_i32(int32_t *p, long long n, int32_t v) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -281,20 +341,27 @@ __global__ void k_fill_u64(unsigned long long *p, long long n, unsigned long lon
 
 // --------------------------------------------------------------------------- seed
 // T(x, {i}) = 0 for every member x of query group i (height 0); activates bit i.
-__global__ void k_seed(const JobConst *jcs, JobState *jss, GraphDev g) {
-  const JobConst &c = jcs[blockIdx.y];
-  JobState *s = &jss[blockIdx.y];
-  if (s->phase != PH_SEED) return;
-  const int li = c.hop ? 0 : 1;
-  const int stride = gridDim.x * blockDim.x;
-  for (int i = 0; i < c.K; i++) {
-    const long long b = g.goffs[c.gid[i]], e = g.goffs[c.gid[i] + 1];
-    for (long long t = b + blockIdx.x * blockDim.x + threadIdx.x; t < e; t += stride) {
-      const int32_t x = g.gverts[t];
-      c.T[(size_t)x * c.H * c.W + (1 << i)] = 0;
-      const unsigned long long old = atomicOr(&c.act[(size_t)li * g.n + x], 1ull << i);
-      if (old == 0) push_vertex(c, s, g, li, x);
+__global__ __launch_bounds__(BLK) void k_seed(const JobConst *jcs, JobState *jss, const Plan *pl, GraphDev g) {
+  __shared__ long long soff[MAXJOBS + 1];
+  const int nj = load_plan(pl, WK_SEED, soff);
+  const long long total = soff[nj];
+  for (long long it = (long long)blockIdx.x * BLK + threadIdx.x; it < total; it += (long long)gridDim.x * BLK) {
+    const int jb = find_job(soff, nj, it);
+    const JobConst &c = jcs[jb];
+    JobState *s = &jss[jb];
+    long long loc = it - soff[jb];
+    int i = 0;
+    for (; i < c.K; i++) {
+      const long long sz = g.goffs[c.gid[i] + 1] - g.goffs[c.gid[i]];
+      if (loc < sz) break;
+      loc -= sz;
     }
+    if (i >= c.K) continue;
+    const int li = c.hop ? 0 : 1;
+    const int32_t x = g.gverts[g.goffs[c.gid[i]] + loc];
+    c.T[(size_t)x * c.H * c.W + (1 << i)] = 0;
+    const unsigned long long old = atomicOr(&c.act[(size_t)li * g.n + x], 1ull << i);
+    if (old == 0) push_vertex(c, s, g, li, x);
   }
 }
 
@@ -354,8 +421,23 @@ struct GrowCounters {
   __device__ void zero() { verts = edges = states = relax = push = atom = 0; }
 };
 
-__device__ __forceinline__ void flush_counters(JobState *s, GrowCounters &k) {
-  // wave reduction then one atomic per wavefront
+// Add a thread's counters to job cj's stats: one atomic per counter per wavefront when
+// all lanes worked for the same job (the common case), per lane otherwise.
+__device__ __forceinline__ void flush_lane(JobState *jss, int cj, const GrowCounters &k) {
+  if (cj < 0) return;
+  JobState *s = &jss[cj];
+  if (k.verts) atomicAdd(&s->st[ST_VERTS], (unsigned long long)k.verts);
+  if (k.edges) atomicAdd(&s->st[ST_EDGES], (unsigned long long)k.edges);
+  if (k.states) atomicAdd(&s->st[ST_STATES], (unsigned long long)k.states);
+  if (k.relax) atomicAdd(&s->st[ST_RELAX], (unsigned long long)k.relax);
+  if (k.push) atomicAdd(&s->st[ST_PUSH], (unsigned long long)k.push);
+  if (k.atom) atomicAdd(&s->st[ST_ATOM], (unsigned long long)k.atom);
+}
+// Called by every lane of the wavefront (converged).
+__device__ void flush_counters(JobState *jss, int cj, GrowCounters &k) {
+  const int first = __shfl(cj, 0, 64);
+  const bool same = __all(cj == first || cj < 0);
+  if (!same || first < 0) { flush_lane(jss, cj, k); return; }
   unsigned long long v[6] = {(unsigned long long)k.verts, (unsigned long long)k.edges,
                              (unsigned long long)k.states, (unsigned long long)k.relax,
                              (unsigned long long)k.push, (unsigned long long)k.atom};
@@ -366,6 +448,7 @@ __device__ __forceinline__ void flush_counters(JobState *s, GrowCounters &k) {
     v[q] = x;
   }
   if ((threadIdx.x & 63) == 0) {
+    JobState *s = &jss[first];
     const int map[6] = {ST_VERTS, ST_EDGES, ST_STATES, ST_RELAX, ST_PUSH, ST_ATOM};
 #pragma unroll
     for (int q = 0; q < 6; q++)
@@ -374,25 +457,28 @@ __device__ __forceinline__ void flush_counters(JobState *s, GrowCounters &k) {
 }
 
 // Thread per vertex (degree < split).
-__global__ __launch_bounds__(BLK) void k_grow_thread(const JobConst *jcs, JobState *jss, GraphDev g) {
-  const JobConst &c = jcs[blockIdx.y];
-  JobState *s = &jss[blockIdx.y];
-  if (s->phase != PH_GROW) return;
-  const int li = cur_list(c, s), lo = nxt_list(c, s);
-  const int cnt = s->cnt[li][0];
-  const int L = s->level;
-  const int RW = c.H * c.W;
-  const int ub = load_ub(s);
+__global__ __launch_bounds__(BLK) void k_grow_thread(const JobConst *jcs, JobState *jss, const Plan *pl, GraphDev g) {
+  __shared__ long long soff[MAXJOBS + 1];
+  const int nj = load_plan(pl, WK_THR, soff);
+  const long long total = soff[nj];
   GrowCounters k;
   k.zero();
-  for (int i = blockIdx.x * BLK + threadIdx.x; i < cnt; i += gridDim.x * BLK) {
-    const int32_t v = c.lst[(size_t)li * g.n + i];
+  int cj = -1;
+  for (long long it = (long long)blockIdx.x * BLK + threadIdx.x; it < total; it += (long long)gridDim.x * BLK) {
+    const int jb = find_job(soff, nj, it);
+    if (jb != cj) { flush_lane(jss, cj, k); k.zero(); cj = jb; }
+    const JobConst &c = jcs[jb];
+    JobState *s = &jss[jb];
+    const int li = cur_list(c, s), lo = nxt_list(c, s);
+    const int L = s->level;
+    const int RW = c.H * c.W;
+    const int32_t v = c.lst[(size_t)li * g.n + (it - soff[jb])];
     const unsigned long long a = c.act[(size_t)li * g.n + v];
     k.verts++;
     int32_t *src = c.T + (size_t)v * RW + (c.hop ? s->depth * c.W : 0);
     unsigned long long gm;
     if (c.hop) { gm = a; k.states += popc((uint32_t)a) + popc((uint32_t)(a >> 32)); }
-    else gm = grow_select(c, s, src, a, L, ub, k.states);
+    else gm = grow_select(c, s, src, a, L, load_ub(s), k.states);
     if (!gm) continue;
     const int nb = popc((uint32_t)gm) + popc((uint32_t)(gm >> 32));
     const int32_t e0 = g.beg[v], e1 = g.beg[v + 1];
@@ -408,32 +494,35 @@ __global__ __launch_bounds__(BLK) void k_grow_thread(const JobConst *jcs, JobSta
       }
     }
   }
-  flush_counters(s, k);
+  flush_counters(jss, cj, k);
 }
 
 // Wavefront per (vertex, edge chunk) for vertices with degree >= split.
-__global__ __launch_bounds__(BLK) void k_grow_wave(const JobConst *jcs, JobState *jss, GraphDev g) {
-  const JobConst &c = jcs[blockIdx.y];
-  JobState *s = &jss[blockIdx.y];
-  if (s->phase != PH_GROW) return;
-  const int li = cur_list(c, s), lo = nxt_list(c, s);
-  const int cnt = s->cnt[li][1];
-  const int L = s->level;
-  const int RW = c.H * c.W;
+__global__ __launch_bounds__(BLK) void k_grow_wave(const JobConst *jcs, JobState *jss, const Plan *pl, GraphDev g) {
+  __shared__ long long soff[MAXJOBS + 1];
+  const int nj = load_plan(pl, WK_WAVE, soff);
+  const long long total = soff[nj];
   const int lane = threadIdx.x & 63;
-  const int ub = load_ub(s);
   GrowCounters k;
   k.zero();
-  const int waves_per_grid = gridDim.x * (BLK / WAVE);
-  for (int it = blockIdx.x * (BLK / WAVE) + (threadIdx.x >> 6); it < cnt; it += waves_per_grid) {
-    const int32_t *item = c.blst + ((size_t)li * c.bcap + it) * 2;
+  int cj = -1;
+  const long long waves_per_grid = (long long)gridDim.x * (BLK / WAVE);
+  for (long long it = (long long)blockIdx.x * (BLK / WAVE) + (threadIdx.x >> 6); it < total; it += waves_per_grid) {
+    const int jb = find_job(soff, nj, it);
+    if (jb != cj) { flush_lane(jss, cj, k); k.zero(); cj = jb; }
+    const JobConst &c = jcs[jb];
+    JobState *s = &jss[jb];
+    const int li = cur_list(c, s), lo = nxt_list(c, s);
+    const int L = s->level;
+    const int RW = c.H * c.W;
+    const int32_t *item = c.blst + ((size_t)li * c.bcap + (it - soff[jb])) * 2;
     const int32_t v = item[0], ch = item[1];
     const unsigned long long a = c.act[(size_t)li * g.n + v];
     int32_t *src = c.T + (size_t)v * RW + (c.hop ? s->depth * c.W : 0);
     unsigned long long gm;
     int nst = 0;
     if (c.hop) gm = a;
-    else gm = grow_select(c, s, src, a, L, ub, nst);  // every lane computes the same mask
+    else gm = grow_select(c, s, src, a, L, load_ub(s), nst);  // every lane computes the same mask
     if (lane == 0) {
       k.verts += ch == 0;
       k.states += ch == 0 ? (c.hop ? popc((uint32_t)a) + popc((uint32_t)(a >> 32)) : nst) : 0;
@@ -454,33 +543,36 @@ __global__ __launch_bounds__(BLK) void k_grow_wave(const JobConst *jcs, JobState
       }
     }
   }
-  flush_counters(s, k);
+  flush_counters(jss, cj, k);
 }
 
-// Zero the active masks of the list just consumed (after both grow kernels ran).
-__global__ void k_clear(const JobConst *jcs, JobState *jss, GraphDev g) {
-  const JobConst &c = jcs[blockIdx.y];
-  JobState *s = &jss[blockIdx.y];
-  int li;
-  if (s->phase == PH_GROW) li = cur_list(c, s);
-  else if (s->phase == PH_HOPCLEAR) li = c.D;
-  else return;
-  const int n0 = s->cnt[li][0], n1 = s->cnt[li][1];
-  for (int i = blockIdx.x * BLK + threadIdx.x; i < n0 + n1; i += gridDim.x * BLK) {
+// Zero the active masks of the list just consumed (after both grow kernels ran), or of
+// the height-D list at the end of a hop level.
+__global__ __launch_bounds__(BLK) void k_clear(const JobConst *jcs, JobState *jss, const Plan *pl, GraphDev g) {
+  __shared__ long long soff[MAXJOBS + 1];
+  const int nj = load_plan(pl, WK_CLEAR, soff);
+  const long long total = soff[nj];
+  for (long long it = (long long)blockIdx.x * BLK + threadIdx.x; it < total; it += (long long)gridDim.x * BLK) {
+    const int jb = find_job(soff, nj, it);
+    const JobConst &c = jcs[jb];
+    const JobState *s = &jss[jb];
+    const int li = s->phase == PH_HOPCLEAR ? c.D : cur_list(c, s);
+    const int n0 = s->cnt[li][0];
+    const long long i = it - soff[jb];
     const int32_t v = i < n0 ? c.lst[(size_t)li * g.n + i] : c.blst[((size_t)li * c.bcap + (i - n0)) * 2];
     c.act[(size_t)li * g.n + v] = 0;
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0 && s->phase == PH_GROW) atomicAdd(&s->st[ST_GROWR], 1ull);
 }
 
 // --------------------------------------------------------------------------- tables
 // After level 1: r1 = UB/2 + 1, group distance lower bounds vv[a][b] and Held-Karp tour
 // tables hk[a][b][X] (cheapest walk from group a through every group of X ending at
 // b in X) and w2[a][X] = min_b hk[a][b][X]. One block per job.
-__global__ __launch_bounds__(BLK) void k_tables(const JobConst *jcs, JobState *jss, GraphDev g) {
-  const JobConst &c = jcs[blockIdx.y];
-  JobState *s = &jss[blockIdx.y];
-  if (s->phase != PH_TABLES || blockIdx.x != 0) return;
+__global__ __launch_bounds__(BLK) void k_tables(const JobConst *jcs, JobState *jss, int njobs, GraphDev g) {
+  if ((int)blockIdx.x >= njobs) return;
+  const JobConst &c = jcs[blockIdx.x];
+  JobState *s = &jss[blockIdx.x];
+  if (s->phase != PH_TABLES) return;
   __shared__ int vv[MAXK * MAXK];
   __shared__ int hk[MAXK * MAXK * MAXW];
   __shared__ int r1s;
@@ -544,15 +636,20 @@ __global__ __launch_bounds__(BLK) void k_tables(const JobConst *jcs, JobState *j
 // A tile of vertices has its rows staged in LDS (coalesced contiguous read), each
 // thread evaluates (vertex, subset) items, new states are activated for growth.
 constexpr int MERGE_LDS_WORDS = 4096;
-__global__ __launch_bounds__(BLK) void k_merge(const JobConst *jcs, JobState *jss, GraphDev g) {
-  const JobConst &c = jcs[blockIdx.y];
-  JobState *s = &jss[blockIdx.y];
-  if (s->phase != PH_MERGE || c.hop) return;
+__global__ __launch_bounds__(BLK) void k_merge(const JobConst *jcs, JobState *jss, const Plan *pl, GraphDev g) {
+  __shared__ long long soff[MAXJOBS + 1];
   __shared__ int32_t rows[MERGE_LDS_WORDS];
   __shared__ unsigned long long amask[512];
   __shared__ int ubs;
-  const int W = c.W, K = c.K, L = s->level, R = c.R;
-  const int TV = min(512, MERGE_LDS_WORDS / W);
+  const int nj = load_plan(pl, WK_MERGE, soff);
+  const long long total = soff[nj];
+  for (long long gt = blockIdx.x; gt < total; gt += gridDim.x) {
+  const int jb = find_job(soff, nj, gt);
+  const JobConst &c = jcs[jb];
+  JobState *s = &jss[jb];
+  if (c.hop) continue;
+  const int W = c.W, K = c.K, L = s->level;
+  const int TV = c.tv;
   const int nL = c.nsub[L];
   const int lo = s->parity ^ 1;
   const bool prune = !(c.flags & JF_NO_PRUNE);
@@ -560,9 +657,8 @@ __global__ __launch_bounds__(BLK) void k_merge(const JobConst *jcs, JobState *js
   const int32_t *w2 = c.tab + K * K * W;
   const int r1 = s->r1;
   int merged = 0;
-  const long long ntiles = ((long long)g.n + TV - 1) / TV;
-  for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    const long long v0 = tile * TV;
+  {
+    const long long v0 = (gt - soff[jb]) * TV;
     const int tv = (int)min((long long)TV, (long long)g.n - v0);
     for (int i = threadIdx.x; i < tv * W; i += BLK) rows[i] = c.T[(size_t)v0 * W + i];
     for (int i = threadIdx.x; i < tv; i += BLK) amask[i] = 0;
@@ -608,24 +704,29 @@ __global__ __launch_bounds__(BLK) void k_merge(const JobConst *jcs, JobState *js
     __syncthreads();
   }
   if (merged) atomicAdd(&s->st[ST_MERGE], (unsigned long long)merged);
+  }
 }
 
 // Hop merge for level L: T(v,p,max(d1,d2)) = min T(v,p1,d1) + T(v,p2,d2), d1 + d2 <= D.
 constexpr int HMERGE_LDS_WORDS = 8192;
-__global__ __launch_bounds__(BLK) void k_merge_hop(const JobConst *jcs, JobState *jss, GraphDev g) {
-  const JobConst &c = jcs[blockIdx.y];
-  JobState *s = &jss[blockIdx.y];
-  if (s->phase != PH_MERGE || !c.hop) return;
+__global__ __launch_bounds__(BLK) void k_merge_hop(const JobConst *jcs, JobState *jss, const Plan *pl, GraphDev g) {
+  __shared__ long long soff[MAXJOBS + 1];
   __shared__ int32_t rows[HMERGE_LDS_WORDS];
   __shared__ unsigned long long amask[2048];
+  const int nj = load_plan(pl, WK_MERGE, soff);
+  const long long total = soff[nj];
+  for (long long gt = blockIdx.x; gt < total; gt += gridDim.x) {
+  const int jb = find_job(soff, nj, gt);
+  const JobConst &c = jcs[jb];
+  JobState *s = &jss[jb];
+  if (!c.hop) continue;
   const int W = c.W, H = c.H, D = c.D, K = c.K, L = s->level;
   const int RW = H * W;
-  const int TV = max(1, min(256, HMERGE_LDS_WORDS / RW));
+  const int TV = c.tv;
   const int nL = c.nsub[L];
   int merged = 0;
-  const long long ntiles = ((long long)g.n + TV - 1) / TV;
-  for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    const long long v0 = tile * TV;
+  {
+    const long long v0 = (gt - soff[jb]) * TV;
     const int tv = (int)min((long long)TV, (long long)g.n - v0);
     for (int i = threadIdx.x; i < tv * RW; i += BLK) rows[i] = c.T[(size_t)v0 * RW + i];
     for (int i = threadIdx.x; i < tv * H; i += BLK) amask[i] = 0;
@@ -672,34 +773,43 @@ __global__ __launch_bounds__(BLK) void k_merge_hop(const JobConst *jcs, JobState
     __syncthreads();
   }
   if (merged) atomicAdd(&s->st[ST_MERGE], (unsigned long long)merged);
+  }
 }
 
 // --------------------------------------------------------------------------- final
-// ans = min over v, d of (T(v, full, d) << 32 | v*H + d).
-__global__ __launch_bounds__(BLK) void k_final(const JobConst *jcs, JobState *jss, GraphDev g) {
-  const JobConst &c = jcs[blockIdx.y];
-  JobState *s = &jss[blockIdx.y];
-  if (s->phase != PH_FINAL) return;
+// ans = min over v, d of (T(v, full, d) << 32 | v*H + d); a work item is a chunk of
+// FINAL_CHUNK rows (v, d) of one job.
+__global__ __launch_bounds__(BLK) void k_final(const JobConst *jcs, JobState *jss, const Plan *pl, GraphDev g) {
+  __shared__ long long soff[MAXJOBS + 1];
   __shared__ unsigned long long bm;
-  if (threadIdx.x == 0) bm = ~0ull;
-  __syncthreads();
-  unsigned long long best = ~0ull;
-  const int H = c.H, W = c.W, R = c.R;
-  for (long long i = (long long)blockIdx.x * BLK + threadIdx.x; i < (long long)g.n * H;
-       i += (long long)gridDim.x * BLK) {
-    const int32_t cst = c.T[(size_t)i * W + R];
-    if (cst < INF) {
-      const unsigned long long key = ((unsigned long long)cst << 32) | (unsigned long long)i;
-      best = key < best ? key : best;
+  const int nj = load_plan(pl, WK_FINAL, soff);
+  const long long total = soff[nj];
+  for (long long gt = blockIdx.x; gt < total; gt += gridDim.x) {
+    const int jb = find_job(soff, nj, gt);
+    const JobConst &c = jcs[jb];
+    if (threadIdx.x == 0) bm = ~0ull;
+    __syncthreads();
+    unsigned long long best = ~0ull;
+    const int W = c.W, R = c.R;
+    const long long rows = (long long)g.n * c.H;
+    const long long r0 = (gt - soff[jb]) * FINAL_CHUNK;
+    const long long r1 = min(rows, r0 + FINAL_CHUNK);
+    for (long long i = r0 + threadIdx.x; i < r1; i += BLK) {
+      const int32_t cst = c.T[(size_t)i * W + R];
+      if (cst < INF) {
+        const unsigned long long key = ((unsigned long long)cst << 32) | (unsigned long long)i;
+        best = key < best ? key : best;
+      }
     }
+    for (int off = 32; off > 0; off >>= 1) {
+      const unsigned long long o = __shfl_xor(best, off, 64);
+      best = o < best ? o : best;
+    }
+    if ((threadIdx.x & 63) == 0 && best != ~0ull) atomicMin(&bm, best);
+    __syncthreads();
+    if (threadIdx.x == 0 && bm != ~0ull) atomicMin(&jss[jb].ans, bm);
+    __syncthreads();
   }
-  for (int off = 32; off > 0; off >>= 1) {
-    const unsigned long long o = __shfl_xor(best, off, 64);
-    best = o < best ? o : best;
-  }
-  if ((threadIdx.x & 63) == 0 && best != ~0ull) atomicMin(&bm, best);
-  __syncthreads();
-  if (threadIdx.x == 0 && bm != ~0ull) atomicMin(&s->ans, bm);
 }
 
 // --------------------------------------------------------------------------- rebuild
@@ -717,10 +827,11 @@ __device__ __forceinline__ bool in_group(const GraphDev &g, int grp, int32_t v) 
   return lo < g.goffs[grp + 1] && g.gverts[lo] == v;
 }
 
-__global__ __launch_bounds__(BLK) void k_rebuild(const JobConst *jcs, JobState *jss, GraphDev g) {
-  const JobConst &c = jcs[blockIdx.y];
-  JobState *s = &jss[blockIdx.y];
-  if (s->phase != PH_REBUILD || blockIdx.x != 0) return;
+__global__ __launch_bounds__(BLK) void k_rebuild(const JobConst *jcs, JobState *jss, int njobs, GraphDev g) {
+  if ((int)blockIdx.x >= njobs) return;
+  const JobConst &c = jcs[blockIdx.x];
+  JobState *s = &jss[blockIdx.x];
+  if (s->phase != PH_REBUILD) return;
   __shared__ int top, nout, err, cv, cp, cd, ccost;
   __shared__ long long found;
   const int H = c.H, W = c.W, D = c.D;

@@ -1,0 +1,93 @@
+"""Batched persistent form and serving: every result equals the single-query result
+(exact integers) and the CPU oracle; trees are valid."""
+import time
+
+import numpy as np
+import pytest
+
+import pyoracle
+from gpu4gst_1_amd import engine
+
+pytestmark = pytest.mark.gpu
+
+
+def _mixed_queries(rmat, n, seed):
+    rng = np.random.default_rng(seed)
+    qs = []
+    for i in range(n):
+        k = int(rng.choice([3, 5, 7]))
+        D = int(rng.choice([-1, -1, 2, 3, 4]))
+        qs.append(engine.Query(rng.choice(rmat.groups.n_groups, k, replace=False).astype(np.int32), D))
+    return qs
+
+
+def _oracle_cost(inst, q):
+    o = pyoracle.prunedpp(inst, q.gids) if q.D < 0 else pyoracle.dprunedpp(inst, q.gids, q.D)
+    return o["tree_cost"] if o["status"] == 0 else None
+
+
+def _assert_same(rmat, inst, q, r, alone):
+    oc = _oracle_cost(inst, q)
+    if oc is None:
+        assert r["status"] == engine.GST_NO_SOLUTION and alone["status"] == engine.GST_NO_SOLUTION
+        return
+    assert r["status"] == engine.GST_OK, r
+    assert r["cost"] == alone["cost"] == oc, (q, r["cost"], alone["cost"], oc)
+    assert r["tree_cost"] == r["cost"]
+    ok, why = engine.validate_tree(rmat.graph, rmat.groups, q.gids, r["edges"], r["cost"], q.D)
+    assert ok, why
+
+
+def test_batch_matches_plain_and_oracle(gpu_engine, rmat, rmat_inst):
+    qs = _mixed_queries(rmat, 24, 1)
+    print("batch of", len(qs), flush=True)
+    res = gpu_engine.solve_batch(qs)
+    for q, r in zip(qs, res):
+        alone = gpu_engine.solve(q.gids, D=q.D)
+        _assert_same(rmat, rmat_inst, q, r, alone)
+
+
+def test_batch_small_budget_serialises(rmat, rmat_inst):
+    # a budget that fits only one k=7 arena at a time forces queuing; results unchanged
+    budget = engine.arena_bytes(rmat.graph.n, 7) + 1
+    with engine.Engine(rmat.graph, rmat.groups, dp_budget_bytes=budget, max_batch=8) as eng:
+        qs = _mixed_queries(rmat, 10, 2)
+        res = eng.solve_batch(qs)
+        for q, r in zip(qs, res):
+            alone = eng.solve(q.gids, D=q.D)
+            _assert_same(rmat, rmat_inst, q, r, alone)
+        info = eng.info()
+        assert info["completed"] >= 10 and info["in_use_bytes"] == 0
+
+
+def test_serving_continuous_admission(gpu_engine, rmat, rmat_inst):
+    """Requests submitted while others are on the device; each result equals the
+    result of the same query alone."""
+    qs = _mixed_queries(rmat, 30, 3)
+    tickets = []
+    for i in range(0, len(qs), 5):
+        tickets += gpu_engine.submit(qs[i:i + 5])
+        time.sleep(0.002)
+    got = {}
+    deadline = time.time() + 60
+    while len(got) < len(tickets) and time.time() < deadline:
+        for t in tickets:
+            if t not in got:
+                r = gpu_engine.poll(t)
+                if r is not None:
+                    got[t] = r
+        time.sleep(0.001)
+    assert len(got) == len(tickets)
+    for q, t in zip(qs, tickets):
+        alone = gpu_engine.solve(q.gids, D=q.D)
+        _assert_same(rmat, rmat_inst, q, got[t], alone)
+
+
+def test_serving_rejects_bad_and_oversized(rmat):
+    with engine.Engine(rmat.graph, rmat.groups, dp_budget_bytes=1 << 20) as eng:
+        t = eng.submit([engine.Query(np.array([0, 1, 2], np.int32)),
+                        engine.Query(np.array([rmat.groups.n_groups], np.int32))])
+        r0 = eng.wait(t[0])
+        r1 = eng.wait(t[1])
+        assert r0["status"] == engine.GST_OVER_BUDGET
+        assert r1["status"] == engine.GST_BAD_INPUT
