@@ -1,0 +1,177 @@
+"""Benchmark: GST queries/s at k=5, batched vs plain, on one MI355X.
+
+Workload: Chung-Lu power-law graph (degree exponent 2.0), 3.07M vertices / 117M
+undirected edges, largest connected component, int32 weights uniform in [1,100],
+20,000 Zipf(1.5)-sized groups in [1,50]; queries of k=5 distinct group ids (seeded).
+
+A step = one batch of --batch queries solved by the batched persistent path
+(gst_solve_batch). value = queries/s over the timed steps (inputs resident in HBM).
+Beside it: the plain single-query path on --plain-queries of the same queries (q/s,
+p50/p99), the grow-kernel roofline (algorithmic bytes / HIP-event time of the grow
+kernels, plain path), and the CPU PrunedDP++ restatement timed on the host cores on a
+bounded sample, whose costs also serve as the parity check of the GPU results.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+import __graft_entry__  # noqa: E402
+
+__graft_entry__.load_package()
+from gpu4gst_1_amd import datasets, engine  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+
+def b_grow(r):
+    """Algorithmic bytes of the grow rounds of one query:
+    8|S| (beg_pos pair) + 8 E_S (adj + weight) + 4|F| (source cost) + 8 R (target cost read
+    + atomic write) + 4 |F_next| (worklist push)."""
+    return (8 * r["grow_vertices"] + 8 * r["grow_edges"] + 4 * r["states_processed"] + 8 * r["grow_relax"]
+            + 4 * r["grow_pushes"])
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--batch", type=int, default=64)
+    ap.add_argument("--k", type=int, default=5)
+    ap.add_argument("--workload", default="cl3m")
+    ap.add_argument("--plain-queries", type=int, default=24)
+    ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU-baseline work")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile-only", action="store_true", help="plain path only (for rocprof runs)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
+                                   "(the path shards by query, see the design notes)"}))
+        sys.exit(2)
+    import torch
+
+    assert torch.cuda.is_available(), "bench needs a GPU"
+    t0 = time.time()
+    wl = datasets.make_workload(args.workload)
+    g = wl.graph
+    print(f"# workload {args.workload}: n={g.n} edges={g.n_edges} groups={wl.groups.n_groups} "
+          f"(generated in {time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
+    need = args.batch * (args.steps + args.warmup)
+    rng = np.random.default_rng(datasets.SEED_BASE + 77)
+    pool = wl.queries[args.k]
+    extra = [rng.choice(wl.groups.n_groups, args.k, replace=False).astype(np.int32)
+             for _ in range(max(0, need - len(pool)))]
+    queries = [np.asarray(q, np.int32) for q in pool] + extra
+    queries = queries[:need]
+
+    eng = engine.Engine(g, wl.groups, max_batch=args.batch)
+    print(f"# opened engine ({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
+
+    # ---- plain path: one query at a time (also gives the grow-kernel roofline)
+    plain = []
+    eng.solve(queries[0])  # warm-up
+    for i, q in enumerate(queries[:args.plain_queries]):
+        plain.append(eng.solve(q))
+    lat = np.array([r["t_total_s"] for r in plain]) * 1e3
+    plain_qps = len(plain) / max(sum(r["t_total_s"] for r in plain), 1e-9)
+    gb = sum(b_grow(r) for r in plain)
+    gt = sum(r["t_grow_s"] for r in plain)
+    grow_gbs = gb / gt / 1e9 if gt > 0 else 0.0
+    print(f"# plain: {plain_qps:.2f} q/s p50 {np.median(lat):.2f} ms p99 {np.percentile(lat, 99):.2f} ms; "
+          f"grow {grow_gbs:.1f} GB/s ({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
+    if args.profile_only:
+        eng.close()
+        return
+
+    # ---- batched path: warmup then timed steps
+    batches = [queries[i * args.batch:(i + 1) * args.batch] for i in range(args.warmup + args.steps)]
+    for b in batches[:args.warmup]:
+        eng.solve_batch(b)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    results = []
+    for b in batches[args.warmup:]:
+        results += eng.solve_batch(b)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t1
+    nq = sum(len(b) for b in batches[args.warmup:])
+    qps = nq / dt
+    blat = np.array([r["t_total_s"] for r in results]) * 1e3
+    ok = sum(r["status"] == engine.GST_OK for r in results)
+    print(f"# batched: {qps:.2f} q/s over {nq} queries in {dt:.2f}s ({ok} solved)", file=sys.stderr, flush=True)
+
+    # ---- parity of batched vs plain on the shared queries
+    by_q = {}
+    for b, rs in zip([q for bb in batches[args.warmup:] for q in bb], results):
+        by_q[tuple(b.tolist())] = rs
+    plain_match = sum(1 for q, r in zip(queries[:args.plain_queries], plain)
+                      if tuple(q.tolist()) in by_q and by_q[tuple(q.tolist())]["cost"] == r["cost"])
+
+    # ---- CPU baseline: PrunedDP++ restatement on the host cores, bounded sample
+    cpu = None
+    parity = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle
+
+        inst = pyoracle.Instance(g, wl.groups)
+        cores = min(os.cpu_count() or 1, 16)
+        sample = np.stack(queries[:max(cores, 1)])
+        tc = time.perf_counter()
+        ores = pyoracle.batch(inst, sample, algo=0, n_threads=cores, time_budget_s=args.cpu_budget)
+        cdt = time.perf_counter() - tc
+        solved = [(i, o) for i, o in enumerate(ores) if o["status"] != -99]
+        cpu = {"value": round(len(solved) / cdt, 4), "unit": "queries/s", "cores": cores, "kind": "port",
+               "sample": f"{len(solved)} queries of k={args.k} on the same graph, one per thread, "
+                         f"{cdt:.1f}s wall"}
+        gpu_cost = {}
+        for q, r in zip(queries[:args.plain_queries], plain):
+            gpu_cost[tuple(q.tolist())] = r["cost"]
+        for q, r in by_q.items():
+            gpu_cost.setdefault(q, r["cost"])
+        checked = [(tuple(sample[i].tolist()), o) for i, o in solved if tuple(sample[i].tolist()) in gpu_cost]
+        agree = sum(1 for q, o in checked if o["tree_cost"] == gpu_cost[q])
+        parity = f"{agree}/{len(checked)}"
+        print(f"# cpu: {cpu['value']} q/s on {cores} threads; parity {parity} ({time.time() - t0:.1f}s)",
+              file=sys.stderr, flush=True)
+
+    out = {
+        "metric": "GST queries/s at k=5 (batched persistent path), p50/p99 ms, bit-exact vs PrunedDP++",
+        "value": round(qps, 3),
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic (seeded Chung-Lu graph, Zipf groups, uniform queries)",
+        "config": {"workload": f"{args.workload}: Chung-Lu beta=2.0, n={g.n}, undirected edges={g.n_edges}, "
+                               f"{wl.groups.n_groups} groups, k={args.k}",
+                   "batch": args.batch, "plain_queries": len(plain)},
+        "p50_ms": round(float(np.median(blat)), 3),
+        "p99_ms": round(float(np.percentile(blat, 99)), 3),
+        "plain": {"qps": round(plain_qps, 3), "p50_ms": round(float(np.median(lat)), 3),
+                  "p99_ms": round(float(np.percentile(lat, 99)), 3)},
+        "batched_vs_plain": round(qps / plain_qps, 3) if plain_qps > 0 else None,
+        "parity_vs_cpu": parity,
+        "parity_batched_vs_plain": f"{plain_match}/{len(plain)}",
+        "roofline": {"bound": "hbm", "achieved": round(grow_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(grow_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_grow_thread + k_grow_wave (plain path, HIP events)"},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    eng.close()
+
+
+if __name__ == "__main__":
+    main()
