@@ -24,7 +24,7 @@ sys.path.insert(0, ROOT)
 import __graft_entry__  # noqa: E402
 
 __graft_entry__.load_package()
-from gpu4gst_1_amd import datasets, engine  # noqa: E402
+from gpu4gst_1_amd import autotune, datasets, engine  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 
@@ -49,6 +49,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU-baseline work")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-only", action="store_true", help="plain path only (for rocprof runs)")
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (library defaults)")
+    ap.add_argument("--stats", action="store_true", help="count finite states (extra arena scan per query)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
@@ -72,12 +74,21 @@ def main():
 
     eng = engine.Engine(g, wl.groups, max_batch=args.batch)
     print(f"# opened engine ({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
+    tuned = None
+    if not args.no_tune:
+        sample = queries[-3:]
+        tuned, _ = autotune.tune(eng, sample, grid={"degree_split": (128, 256, 512), "chunk_edges": (4096, 16384),
+                                                   "near_far_delta": (0,)}, budget_s=30.0)
+        print(f"# autotuned {tuned} ({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
+    flags = engine.F_STATS if args.stats else 0
 
-    # ---- plain path: one query at a time (also gives the grow-kernel roofline)
+    # ---- plain path: one query at a time (also gives the grow-kernel roofline), on the
+    # first queries of the first timed batch so that both paths see the same queries
     plain = []
-    eng.solve(queries[0])  # warm-up
-    for i, q in enumerate(queries[:args.plain_queries]):
-        plain.append(eng.solve(q))
+    plain_q = queries[args.warmup * args.batch:][:args.plain_queries]
+    eng.solve(plain_q[0], flags=flags)  # warm-up
+    for q in plain_q:
+        plain.append(eng.solve(q, flags=flags))
     lat = np.array([r["t_total_s"] for r in plain]) * 1e3
     plain_qps = len(plain) / max(sum(r["t_total_s"] for r in plain), 1e-9)
     gb = sum(b_grow(r) for r in plain)
@@ -110,7 +121,7 @@ def main():
     by_q = {}
     for b, rs in zip([q for bb in batches[args.warmup:] for q in bb], results):
         by_q[tuple(b.tolist())] = rs
-    plain_match = sum(1 for q, r in zip(queries[:args.plain_queries], plain)
+    plain_match = sum(1 for q, r in zip(plain_q, plain)
                       if tuple(q.tolist()) in by_q and by_q[tuple(q.tolist())]["cost"] == r["cost"])
 
     # ---- CPU baseline: PrunedDP++ restatement on the host cores, bounded sample
@@ -122,7 +133,7 @@ def main():
 
         inst = pyoracle.Instance(g, wl.groups)
         cores = min(os.cpu_count() or 1, 16)
-        sample = np.stack(queries[:max(cores, 1)])
+        sample = np.stack(plain_q[:max(cores, 1)])
         tc = time.perf_counter()
         ores = pyoracle.batch(inst, sample, algo=0, n_threads=cores, time_budget_s=args.cpu_budget)
         cdt = time.perf_counter() - tc
@@ -131,7 +142,7 @@ def main():
                "sample": f"{len(solved)} queries of k={args.k} on the same graph, one per thread, "
                          f"{cdt:.1f}s wall"}
         gpu_cost = {}
-        for q, r in zip(queries[:args.plain_queries], plain):
+        for q, r in zip(plain_q, plain):
             gpu_cost[tuple(q.tolist())] = r["cost"]
         for q, r in by_q.items():
             gpu_cost.setdefault(q, r["cost"])
@@ -156,7 +167,7 @@ def main():
         "data": "synthetic (seeded Chung-Lu graph, Zipf groups, uniform queries)",
         "config": {"workload": f"{args.workload}: Chung-Lu beta=2.0, n={g.n}, undirected edges={g.n_edges}, "
                                f"{wl.groups.n_groups} groups, k={args.k}",
-                   "batch": args.batch, "plain_queries": len(plain)},
+                   "batch": args.batch, "plain_queries": len(plain), "tuning": tuned},
         "p50_ms": round(float(np.median(blat)), 3),
         "p99_ms": round(float(np.percentile(blat, 99)), 3),
         "plain": {"qps": round(plain_qps, 3), "p50_ms": round(float(np.median(lat)), 3),

@@ -1,0 +1,62 @@
+"""Autotuner of the grow phase: degree split (thread vs wavefront per vertex), edge chunk
+per wavefront item, and near-far step, chosen per (graph, k) by timing a sample of
+queries on the device.
+
+The tuning changes only the schedule of the same exact DP, so every candidate returns
+the same optimum; the tuner checks that on its sample and refuses a candidate that
+disagrees (it would be a bug).
+"""
+from __future__ import annotations
+
+import itertools
+import time
+
+import numpy as np
+
+DEFAULT_GRID = {
+    "degree_split": (64, 128, 256, 512),
+    "chunk_edges": (1024, 4096, 16384),
+    "near_far_delta": (0, 64),
+}
+
+
+def tune(eng, queries, D=-1, grid=None, repeats=1, budget_s=60.0, log=None):
+    """Time every grid point on `queries` (plain path) and apply the fastest.
+
+    Returns (best_config dict, table of (config, seconds per query))."""
+    grid = grid or DEFAULT_GRID
+    keys = list(grid)
+    table = []
+    ref = None
+    t_start = time.perf_counter()
+    best = None
+    for vals in itertools.product(*(grid[k] for k in keys)):
+        cfg = dict(zip(keys, vals))
+        eng.set_tuning(cfg["degree_split"], cfg["chunk_edges"], cfg["near_far_delta"])
+        eng.solve(queries[0], D=D)  # warm (allocations, code paths)
+        t = time.perf_counter()
+        costs = []
+        for _ in range(repeats):
+            costs = [eng.solve(q, D=D)["cost"] for q in queries]
+        dt = (time.perf_counter() - t) / (repeats * len(queries))
+        if ref is None:
+            ref = costs
+        elif costs != ref:
+            raise RuntimeError(f"tuning {cfg} changed the optimum: {costs} vs {ref}")
+        table.append((cfg, dt))
+        if log:
+            log(f"autotune {cfg} {dt * 1e3:.2f} ms/query")
+        if best is None or dt < best[1]:
+            best = (cfg, dt)
+        if time.perf_counter() - t_start > budget_s:
+            break
+    cfg = best[0]
+    eng.set_tuning(cfg["degree_split"], cfg["chunk_edges"], cfg["near_far_delta"])
+    return cfg, table
+
+
+def degree_profile(graph):
+    """Summary used to key tuned configurations: vertices, arcs, degree quantiles."""
+    d = graph.degrees()
+    return {"n": int(graph.n), "m": int(graph.m), "p50": float(np.median(d)), "p99": float(np.percentile(d, 99)),
+            "max": int(d.max())}

@@ -57,8 +57,15 @@ struct Slot {
   unsigned long long *act = nullptr;
   int32_t *lst = nullptr, *blst = nullptr, *tab = nullptr, *stk = nullptr, *out = nullptr;
   size_t bytes = 0;
-  int nl = 0;
+  int nact = 0, nlst = 0, nblst = 0;  // mask arrays, thread lists, wavefront lists
 };
+
+// Non-hop: near masks 0/1 + far-pile mask 2; near lists 0/1 + far piles 2/3 (thread
+// items only); wavefront lists 0/1. Hop: one of each per height.
+inline void slot_shape(int D, int &nact, int &nlst, int &nblst) {
+  if (D >= 0) { nact = nlst = nblst = D + 1; }
+  else { nact = 3; nlst = 4; nblst = 2; }
+}
 
 constexpr int SCAP = 1 << 16;  // rebuild stack entries
 constexpr int ECAP = 1 << 15;  // tree edges returned
@@ -72,6 +79,7 @@ struct gst_ctx {
   int64_t m = 0;
   GraphDev g{};
   int32_t *d_beg = nullptr, *d_adj = nullptr, *d_w = nullptr, *d_gverts = nullptr;
+  int32_t *d_nchunk = nullptr;  // per vertex: wavefront chunks (0 = thread item)
   long long *d_goffs = nullptr;
   std::vector<int64_t> h_goffs;
   std::vector<int32_t> h_deg;
@@ -79,7 +87,7 @@ struct gst_ctx {
   int64_t graph_bytes = 0;
   long long bcap = 0;  // wavefront items one worklist can hold
   gst_opts opts{};
-  int32_t split = 32, chunk = 512;
+  int32_t split = 256, chunk = 4096, delta = 0;
   // single-query path
   std::mutex solve_mu;
   Slot plain;
@@ -105,11 +113,12 @@ int64_t arena_bytes_for(int64_t n, int K, int D) {
 // bytes of a slot for (K, D)
 size_t slot_bytes(const gst_ctx *c, int K, int D) {
   const int H = D >= 0 ? D + 1 : 1, W = 1 << K;
-  const int nl = D >= 0 ? H : 2;
+  int nact, nlst, nblst;
+  slot_shape(D, nact, nlst, nblst);
   size_t b = (size_t)c->n * W * H * 4;                 // arena
-  b += (size_t)nl * c->n * 8;                           // act
-  b += (size_t)nl * c->n * 4;                           // thread lists
-  b += (size_t)nl * c->bcap * 8;                        // wavefront lists
+  b += (size_t)nact * c->n * 8;                         // masks
+  b += (size_t)nlst * c->n * 4;                         // thread lists
+  b += (size_t)nblst * c->bcap * 8;                     // wavefront lists
   b += (size_t)(MAXK * MAXK * MAXW + MAXK * MAXW + MAXK * MAXK) * 4;  // tables
   b += (size_t)SCAP * 12 + (size_t)ECAP * 12;
   return b;
@@ -125,7 +134,7 @@ int slot_alloc(gst_ctx *c, Slot &s, int K, int D) {
   s.K = K; s.D = D;
   s.H = D >= 0 ? D + 1 : 1;
   s.W = 1 << K;
-  s.nl = D >= 0 ? s.H : 2;
+  slot_shape(D, s.nact, s.nlst, s.nblst);
   s.arena_words = (size_t)c->n * s.W * s.H;
   s.bytes = slot_bytes(c, K, D);
   char *base = nullptr;
@@ -139,9 +148,9 @@ int slot_alloc(gst_ctx *c, Slot &s, int K, int D) {
   size_t off = 0;
   auto take = [&](size_t bytes) { char *p = base + off; off += (bytes + 255) & ~(size_t)255; return p; };
   s.T = (int32_t *)take(s.arena_words * 4);
-  s.act = (unsigned long long *)take((size_t)s.nl * c->n * 8);
-  s.lst = (int32_t *)take((size_t)s.nl * c->n * 4);
-  s.blst = (int32_t *)take((size_t)s.nl * c->bcap * 8);
+  s.act = (unsigned long long *)take((size_t)s.nact * c->n * 8);
+  s.lst = (int32_t *)take((size_t)s.nlst * c->n * 4);
+  s.blst = (int32_t *)take((size_t)s.nblst * c->bcap * 8);
   s.tab = (int32_t *)take((size_t)(MAXK * MAXK * MAXW + MAXK * MAXW + MAXK * MAXK) * 4);
   s.stk = (int32_t *)take((size_t)SCAP * 12);
   s.out = (int32_t *)take((size_t)ECAP * 12);
@@ -152,9 +161,9 @@ int slot_alloc(gst_ctx *c, Slot &s, int K, int D) {
     if (e != hipSuccess) { (void)hipGetLastError(); set_err("hipMalloc(slot) failed"); s = Slot(); return GST_OVER_BUDGET; }
     off = 0;
     s.T = (int32_t *)take(s.arena_words * 4);
-    s.act = (unsigned long long *)take((size_t)s.nl * c->n * 8);
-    s.lst = (int32_t *)take((size_t)s.nl * c->n * 4);
-    s.blst = (int32_t *)take((size_t)s.nl * c->bcap * 8);
+    s.act = (unsigned long long *)take((size_t)s.nact * c->n * 8);
+    s.lst = (int32_t *)take((size_t)s.nlst * c->n * 4);
+    s.blst = (int32_t *)take((size_t)s.nblst * c->bcap * 8);
     s.tab = (int32_t *)take((size_t)(MAXK * MAXK * MAXW + MAXK * MAXW + MAXK * MAXK) * 4);
     s.stk = (int32_t *)take((size_t)SCAP * 12);
     s.out = (int32_t *)take((size_t)ECAP * 12);
@@ -189,7 +198,8 @@ void fill_jobconst(const gst_ctx *c, const Slot &s, const Query &q, JobConst &jc
   jc.stk = s.stk; jc.out = s.out;
   jc.bcap = c->bcap; jc.scap = SCAP; jc.ecap = ECAP;
   jc.K = q.K; jc.D = q.D; jc.H = q.D >= 0 ? q.D + 1 : 1; jc.W = 1 << q.K; jc.R = jc.W - 1;
-  jc.hop = q.D >= 0; jc.flags = q.flags; jc.nl = jc.hop ? jc.H : 2;
+  jc.hop = q.D >= 0; jc.flags = q.flags; jc.nl = s.nlst;
+  jc.delta = q.D >= 0 ? 0 : c->delta;
   for (int i = 0; i < q.K; i++) jc.gid[i] = q.gids[i];
   long long members = 0;
   for (int i = 0; i < q.K; i++) members += c->h_goffs[q.gids[i] + 1] - c->h_goffs[q.gids[i]];
@@ -211,6 +221,9 @@ void init_jobstate(JobState &js) {
   js.ub = INF;
   js.r1 = LB_CAP;
   js.ans = ~0ull;
+  js.theta = INF;
+  js.far_min = INF;
+  js.minc = INF;
 }
 
 inline unsigned grid_for(long long items, long long per_block, unsigned cap) {
@@ -223,7 +236,7 @@ inline unsigned grid_for(long long items, long long per_block, unsigned cap) {
 // Launch the init kernels of a job (arena = INF, masks = 0) on stream st.
 void launch_job_init(hipStream_t st, const Slot &s, const gst_ctx *c) {
   k_fill_i32<<<grid_for((long long)s.arena_words / 4, BLK, 8192), BLK, 0, st>>>(s.T, (long long)s.arena_words, INF);
-  k_fill_u64<<<grid_for((long long)s.nl * c->n, BLK, 4096), BLK, 0, st>>>(s.act, (long long)s.nl * c->n, 0ull);
+  k_fill_u64<<<grid_for((long long)s.nact * c->n, BLK, 4096), BLK, 0, st>>>(s.act, (long long)s.nact * c->n, 0ull);
 }
 
 void finish_result(const gst_ctx *c, const Query &q, const JobState &js, const std::vector<int32_t> &edges,
@@ -346,8 +359,9 @@ int gst_open(const gst_csr *csr, const gst_groups *groups, const gst_opts *opts,
   c->n = n;
   c->m = m;
   c->n_groups = groups->n_groups;
-  c->split = c->opts.degree_split > 0 ? c->opts.degree_split : 32;
-  c->chunk = c->opts.chunk_edges > 0 ? ((c->opts.chunk_edges + 63) / 64) * 64 : 512;
+  c->split = c->opts.degree_split > 0 ? c->opts.degree_split : 256;
+  c->chunk = c->opts.chunk_edges > 0 ? ((c->opts.chunk_edges + 63) / 64) * 64 : 4096;
+  c->delta = c->opts.near_far_delta > 0 ? c->opts.near_far_delta : 0;
   c->max_batch = c->opts.max_batch > 0 ? c->opts.max_batch : 64;
   std::vector<int32_t> beg32(n + 1);
   c->h_deg.resize(n);
@@ -360,6 +374,7 @@ int gst_open(const gst_csr *csr, const gst_groups *groups, const gst_opts *opts,
   HIPCHK(hipMalloc((void **)&c->d_w, sizeof(int32_t) * std::max<int64_t>(m, 1)));
   HIPCHK(hipMalloc((void **)&c->d_goffs, sizeof(long long) * c->h_goffs.size()));
   HIPCHK(hipMalloc((void **)&c->d_gverts, sizeof(int32_t) * std::max<int64_t>(total, 1)));
+  HIPCHK(hipMalloc((void **)&c->d_nchunk, sizeof(int32_t) * n));
   HIPCHK(hipMemcpy(c->d_beg, beg32.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice));
   if (m) {
     HIPCHK(hipMemcpy(c->d_adj, csr->adj, sizeof(int32_t) * m, hipMemcpyHostToDevice));
@@ -379,8 +394,8 @@ int gst_open(const gst_csr *csr, const gst_groups *groups, const gst_opts *opts,
   HIPCHK(hipEventCreate(&c->ev0));
   HIPCHK(hipEventCreate(&c->ev1));
   c->g.beg = c->d_beg; c->g.adj = c->d_adj; c->g.w = c->d_w;
-  c->g.goffs = c->d_goffs; c->g.gverts = c->d_gverts; c->g.n = n;
-  int rc = gst_set_tuning(c.get(), c->split, c->chunk);
+  c->g.goffs = c->d_goffs; c->g.gverts = c->d_gverts; c->g.nchunk = c->d_nchunk; c->g.n = n;
+  int rc = gst_set_tuning(c.get(), c->split, c->chunk, c->delta);
   if (rc) return rc;
   rc = batch_open(c.get());
   if (rc) return rc;
@@ -388,12 +403,18 @@ int gst_open(const gst_csr *csr, const gst_groups *groups, const gst_opts *opts,
   return GST_OK;
 }
 
-int gst_set_tuning(gst_ctx *c, int32_t degree_split, int32_t chunk_edges) {
-  if (!c || degree_split < 1 || chunk_edges < 64) { set_err("bad tuning"); return GST_BAD_INPUT; }
+int gst_set_tuning(gst_ctx *c, int32_t degree_split, int32_t chunk_edges, int32_t near_far_delta) {
+  if (!c || degree_split < 1 || chunk_edges < 64 || near_far_delta < 0) { set_err("bad tuning"); return GST_BAD_INPUT; }
+  c->delta = near_far_delta;
   chunk_edges = ((chunk_edges + 63) / 64) * 64;
   long long bcap = 0;
+  std::vector<int32_t> nch(c->n, 0);
   for (int32_t v = 0; v < c->n; v++)
-    if (c->h_deg[v] >= degree_split) bcap += (c->h_deg[v] + chunk_edges - 1) / chunk_edges;
+    if (c->h_deg[v] >= degree_split) {
+      nch[v] = (c->h_deg[v] + chunk_edges - 1) / chunk_edges;
+      bcap += nch[v];
+    }
+  HIPCHK(hipMemcpy(c->d_nchunk, nch.data(), sizeof(int32_t) * c->n, hipMemcpyHostToDevice));
   bcap = std::max<long long>(bcap, 1);
   if (bcap > c->bcap) {
     // slots sized for the old capacity must be rebuilt
@@ -464,6 +485,7 @@ int gst_solve(gst_ctx *c, const int32_t *gids, int32_t k, int32_t D, int32_t fla
         const int li = q.D >= 0 ? h.depth : h.parity;
         const int n0 = h.cnt[li][0], n1 = h.cnt[li][1];
         HIPCHK(hipEventRecord(c->ev0, c->st));
+        if (q.D < 0) k_select<<<grid_for(n0 + n1, BLK, 1 << 16), BLK, 0, c->st>>>(c->d_jc, c->d_js, pl, g);
         if (n0) k_grow_thread<<<grid_for(n0, BLK, 1 << 20), BLK, 0, c->st>>>(c->d_jc, c->d_js, pl, g);
         if (n1) k_grow_wave<<<grid_for(n1, BLK / WAVE, 1 << 20), BLK, 0, c->st>>>(c->d_jc, c->d_js, pl, g);
         HIPCHK(hipEventRecord(c->ev1, c->st));
@@ -471,6 +493,9 @@ int gst_solve(gst_ctx *c, const int32_t *gids, int32_t k, int32_t D, int32_t fla
         k_clear<<<grid_for(n0 + n1, BLK, 1 << 16), BLK, 0, c->st>>>(c->d_jc, c->d_js, pl, g);
         break;
       }
+      case PH_FARSPLIT:
+        k_farsplit<<<grid_for(h.cnt[FAR_LIST + h.fpar][0], BLK, 1 << 16), BLK, 0, c->st>>>(c->d_jc, c->d_js, pl, g);
+        break;
       case PH_HOPCLEAR: {
         const int n0 = h.cnt[q.D][0], n1 = h.cnt[q.D][1];
         k_clear<<<grid_for(n0 + n1, BLK, 1 << 16), BLK, 0, c->st>>>(c->d_jc, c->d_js, pl, g);
@@ -543,7 +568,7 @@ void gst_close(gst_ctx *c) {
   batch_close(c);
   slot_free(c->plain);
   (void)hipFree(c->d_beg); (void)hipFree(c->d_adj); (void)hipFree(c->d_w);
-  (void)hipFree(c->d_goffs); (void)hipFree(c->d_gverts);
+  (void)hipFree(c->d_goffs); (void)hipFree(c->d_gverts); (void)hipFree(c->d_nchunk);
   (void)hipFree(c->d_jc); (void)hipFree(c->d_js); (void)hipHostFree(c->h_js);
   (void)hipFree(c->d_count);
   (void)hipFree(c->d_plan);

@@ -51,7 +51,7 @@ constexpr int FINAL_CHUNK = 1024;     // arena rows per k_final work item
 
 enum Phase : int32_t {
   PH_INIT = 0, PH_SEED = 1, PH_GROW = 2, PH_TABLES = 3, PH_MERGE = 4,
-  PH_FINAL = 5, PH_REBUILD = 6, PH_HOPCLEAR = 7, PH_DONE = 8
+  PH_FINAL = 5, PH_REBUILD = 6, PH_HOPCLEAR = 7, PH_DONE = 8, PH_FARSPLIT = 9
 };
 
 enum Stat : int32_t {
@@ -78,6 +78,7 @@ struct JobConst {
   int32_t K, D, H, W, R, hop, flags, nl;
   int32_t seed_total;         // members of the query groups (seed work items)
   int32_t tv;                 // vertices per merge tile
+  int32_t delta;              // near-far step of the grow frontier (0 = off)
   int32_t gid[8];
   uint8_t nsub[8];
   uint8_t sub[8][64];         // sub[L][j]: j-th subset of popcount L, ascending
@@ -89,12 +90,22 @@ struct alignas(16) JobState {
   int32_t cnt[MAXLISTS][2];   // [list][0 = thread items, 1 = wavefront items]
   int32_t ub, r1, err, n_out;
   unsigned long long ans;     // (cost << 32) | (v*H + d), min-reduced
-  int32_t rounds, dp_rounds, dirty, pad1;
+  int32_t rounds, dp_rounds;
+  int32_t theta;              // near-far threshold: states above it wait in the far pile
+  int32_t far_min;            // min cost deferred to the far pile this round
+  int32_t fpar;               // far pile parity (lists 2 + fpar)
+  int32_t minc;               // min cost activated by the last merge (level start)
+  int32_t pad2, pad3;
   unsigned long long st[8];
 };
 
+// Non-hop list indices: near frontier 0/1 (parity), far pile 2/3 (fpar); mask arrays:
+// near 0/1, far pile 2.
+constexpr int FAR_LIST = 2;
+constexpr int FAR_ACT = 2;
+
 // Work items of one round, flattened over jobs: off[kind][j] = first item of job j.
-enum WorkKind : int32_t { WK_SEED = 0, WK_THR = 1, WK_WAVE = 2, WK_CLEAR = 3, WK_MERGE = 4, WK_FINAL = 5, NWK = 6 };
+enum WorkKind : int32_t { WK_SEED = 0, WK_THR = 1, WK_WAVE = 2, WK_CLEAR = 3, WK_MERGE = 4, WK_FINAL = 5, WK_FAR = 6, NWK = 7 };
 struct Plan {
   int32_t njobs, pad;
   long long off[NWK][MAXJOBS + 1];
@@ -106,6 +117,7 @@ struct GraphDev {
   const int32_t *w;     // [m]
   const long long *goffs;
   const int32_t *gverts;
+  const int32_t *nchunk; // [n]: 0 if deg < split (thread item), else ceil(deg / chunk)
   int32_t n;
   int32_t split;        // degree split thread/wavefront
   int32_t chunk;        // edges per wavefront item
@@ -218,7 +230,7 @@ __device__ void step_job(const JobConst &c, JobState &s) {
     case PH_SEED:
       if (c.K == 1) { s.phase = PH_FINAL; return; }
       if (c.hop) { s.depth = 0; goto hop_entry; }
-      goto grow_entry;
+      goto level_start;
     case PH_GROW:
       if (c.hop) { s.depth++; goto hop_entry; }
       goto grow_entry;
@@ -229,6 +241,10 @@ __device__ void step_job(const JobConst &c, JobState &s) {
     case PH_MERGE:
       if (s.level == c.K) { s.phase = PH_FINAL; return; }
       if (c.hop) { s.depth = 0; goto hop_entry; }
+      goto level_start;
+    case PH_FARSPLIT:
+      s.cnt[FAR_LIST + s.fpar][0] = 0;  // consumed; the re-deferred rest is in the other pile
+      s.fpar ^= 1;
       goto grow_entry;
     case PH_HOPCLEAR:
       for (int i = 0; i < MAXLISTS; i++) { s.cnt[i][0] = 0; s.cnt[i][1] = 0; }
@@ -247,8 +263,20 @@ __device__ void step_job(const JobConst &c, JobState &s) {
       s.phase = PH_DONE;
       return;
   }
+level_start:
+  s.theta = c.delta > 0 ? (s.minc < INF ? s.minc : 0) + c.delta : INF;
+  s.minc = INF;
+  s.far_min = INF;
 grow_entry: {
     const int nx = s.parity ^ 1;
+    if (s.cnt[nx][0] + s.cnt[nx][1] == 0 && s.cnt[FAR_LIST + s.fpar][0] > 0) {
+      // near frontier drained: raise the threshold past the cheapest waiting state
+      s.cnt[s.parity][0] = 0; s.cnt[s.parity][1] = 0;
+      s.theta = (s.far_min > s.theta ? s.far_min : s.theta) + c.delta;
+      s.far_min = INF;
+      s.phase = PH_FARSPLIT;
+      return;
+    }
     if (s.cnt[nx][0] + s.cnt[nx][1] == 0) {  // level converged
       s.cnt[s.parity][0] = 0; s.cnt[s.parity][1] = 0;
       if (s.level == 1) { s.phase = PH_TABLES; return; }
@@ -282,7 +310,7 @@ __global__ __launch_bounds__(BLK) void k_step(const JobConst *jcs, JobState *jss
     JobState &s = jss[j];
     step_job(c, s);
     const int ph = s.phase;
-    long long k[NWK] = {0, 0, 0, 0, 0, 0};
+    long long k[NWK] = {0, 0, 0, 0, 0, 0, 0};
     if (ph == PH_SEED) k[WK_SEED] = c.seed_total;
     if (ph == PH_GROW) {
       const int li = c.hop ? s.depth : s.parity;
@@ -293,6 +321,7 @@ __global__ __launch_bounds__(BLK) void k_step(const JobConst *jcs, JobState *jss
     if (ph == PH_HOPCLEAR) k[WK_CLEAR] = s.cnt[c.D][0] + s.cnt[c.D][1];
     if (ph == PH_MERGE) k[WK_MERGE] = ((long long)n + c.tv - 1) / c.tv;
     if (ph == PH_FINAL) k[WK_FINAL] = ((long long)n * c.H + FINAL_CHUNK - 1) / FINAL_CHUNK;
+    if (ph == PH_FARSPLIT) k[WK_FAR] = s.cnt[FAR_LIST + s.fpar][0];
 #pragma unroll
     for (int q = 0; q < NWK; q++) cnt[q][j] = k[q];
   }
@@ -370,7 +399,8 @@ __global__ __launch_bounds__(BLK) void k_seed(const JobConst *jcs, JobState *jss
 // return the subset of bits that may grow. Non-hop only.
 __device__ __forceinline__ unsigned long long grow_select(const JobConst &c, JobState *s,
                                                           const int32_t *row, unsigned long long a,
-                                                          int L, int ub, int &n_states) {
+                                                          int L, int ub, int &n_states, int theta,
+                                                          unsigned long long &defer, int &dmin) {
   const int32_t *hk = c.tab;
   const int32_t *w2 = c.tab + c.K * c.K * c.W;
   const bool prune = !(c.flags & JF_NO_PRUNE);
@@ -384,41 +414,59 @@ __device__ __forceinline__ unsigned long long grow_select(const JobConst &c, Job
     const int cst = row[p];
     if (cst >= INF) continue;
     const int cand = ub_candidate(row, p, cst, L, c);
-    if (cand < ubl) { ubl = cand; atomicMin(&s->ub, cand); }
+    if (cand < ubl) {
+      const int cur = load_ub(s);
+      if (cand < cur) atomicMin(&s->ub, cand);
+      ubl = cand < cur ? cand : cur;
+    }
     if (prune) {
       if (cst > (ubl >> 1)) continue;
       if (L > 1 && cst + lb_tail(row, s->r1, p, c, hk, w2) > ubl) continue;
+    }
+    if (cst > theta) {  // near-far: wait in the far pile
+      defer |= 1ull << j;
+      dmin = cst < dmin ? cst : dmin;
+      continue;
     }
     gm |= 1ull << j;
   }
   return gm;
 }
 
-// Relax edge (v -> u, wt) for every bit of gm: atomicMin on u's states; returns the
-// bits that improved.
-__device__ __forceinline__ unsigned long long relax_edge(int32_t *Tsrc, int32_t *Tdst,
-                                                         const JobConst &c, int L,
-                                                         unsigned long long gm, int32_t wt,
-                                                         int &n_atom) {
-  unsigned long long imp = 0;
-  while (gm) {
-    const int j = __builtin_ctzll(gm);
-    gm &= gm - 1;
-    const int p = c.sub[L][j];
-    int nc = Tsrc[p] + wt;
-    nc = nc < INF ? nc : INF;
-    if (nc < Tdst[p]) {
-      n_atom++;
-      const int old = atomicMin(&Tdst[p], nc);
-      if (nc < old) imp |= 1ull << j;
+// Up to RG active subsets of one source vertex are relaxed together over each edge: the
+// RG target loads of an edge are issued back to back, an improvement is detected
+// against the loaded value, and the atomicMin is fire-and-forget (its result is unused,
+// so no round trip). A concurrent update that got lower first only makes the activation
+// redundant: activating whenever cost < loaded value covers every real improvement.
+constexpr int RG = 8;
+struct RelaxGroup {
+  int n;
+  int p[RG];   // subset ids
+  int c[RG];   // source costs
+  int j[RG];   // bit within the level
+};
+
+__device__ __forceinline__ void take_group(const JobConst &c, int L, const int32_t *src,
+                                           unsigned long long &gm, RelaxGroup &G) {
+  G.n = 0;
+#pragma unroll
+  for (int i = 0; i < RG; i++) {
+    G.p[i] = 0; G.c[i] = INF; G.j[i] = 0;
+    if (gm) {
+      const int j = __builtin_ctzll(gm);
+      gm &= gm - 1;
+      G.p[i] = c.sub[L][j];
+      G.c[i] = src[G.p[i]];
+      G.j[i] = j;
+      G.n = i + 1;
     }
   }
-  return imp;
 }
 
 struct GrowCounters {
   int verts, edges, states, relax, push, atom;
-  __device__ void zero() { verts = edges = states = relax = push = atom = 0; }
+  int dmin;  // cheapest state deferred to the far pile
+  __device__ void zero() { verts = edges = states = relax = push = atom = 0; dmin = INF; }
 };
 
 // Add a thread's counters to job cj's stats: one atomic per counter per wavefront when
@@ -432,12 +480,18 @@ __device__ __forceinline__ void flush_lane(JobState *jss, int cj, const GrowCoun
   if (k.relax) atomicAdd(&s->st[ST_RELAX], (unsigned long long)k.relax);
   if (k.push) atomicAdd(&s->st[ST_PUSH], (unsigned long long)k.push);
   if (k.atom) atomicAdd(&s->st[ST_ATOM], (unsigned long long)k.atom);
+  if (k.dmin < INF) atomicMin(&s->far_min, k.dmin);
 }
 // Called by every lane of the wavefront (converged).
 __device__ void flush_counters(JobState *jss, int cj, GrowCounters &k) {
   const int first = __shfl(cj, 0, 64);
   const bool same = __all(cj == first || cj < 0);
   if (!same || first < 0) { flush_lane(jss, cj, k); return; }
+  int dm = k.dmin;
+  for (int off = 32; off > 0; off >>= 1) {
+    const int o = __shfl_xor(dm, off, 64);
+    dm = o < dm ? o : dm;
+  }
   unsigned long long v[6] = {(unsigned long long)k.verts, (unsigned long long)k.edges,
                              (unsigned long long)k.states, (unsigned long long)k.relax,
                              (unsigned long long)k.push, (unsigned long long)k.atom};
@@ -453,59 +507,371 @@ __device__ void flush_counters(JobState *jss, int cj, GrowCounters &k) {
 #pragma unroll
     for (int q = 0; q < 6; q++)
       if (v[q]) atomicAdd(&s->st[map[q]], v[q]);
+    if (dm < INF) atomicMin(&s->far_min, dm);
   }
 }
 
-// Thread per vertex (degree < split).
+__device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
+
+// Wavefront-aggregated append of cnt items (cnt = 0: nothing) to the counter *ctr that
+// every active lane shares; one atomicAdd per wavefront. Must be called converged.
+__device__ __forceinline__ int wave_append(int *ctr, int cnt) {
+  const int lane = lane_id();
+  int x = cnt;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int y = __shfl_up(x, off, 64);
+    if (lane >= off) x += y;
+  }
+  const int total = __shfl(x, 63, 64);
+  int base = 0;
+  if (lane == 63 && total) base = atomicAdd(ctr, total);
+  base = __shfl(base, 63, 64);
+  return base + x - cnt;
+}
+
+// Per-wavefront worklist staging in LDS: pushes are buffered and appended to the global
+// list with one atomicAdd per flush instead of one per step. Every wavefront of every
+// job hits the same two list counters, so unbuffered pushes serialise on them.
+constexpr int PB_SMALL = 256;  // thread items buffered per wavefront
+constexpr int PB_BIG = 64;     // wavefront-item vertices buffered per wavefront
+struct WavePush {
+  int ns, nb;     // buffered entries (wave-uniform)
+  int jb, lo;     // owner job and list of the buffered entries (-1: none)
+  int32_t *sm;    // LDS [PB_SMALL]
+  int32_t *bv;    // LDS [PB_BIG] vertices
+  int32_t *bn;    // LDS [PB_BIG] chunk counts
+};
+
+__device__ __forceinline__ void wp_init(WavePush &b, int32_t *lds_small, int32_t *lds_big) {
+  const int w = threadIdx.x >> 6;
+  b.ns = b.nb = 0;
+  b.jb = -1;
+  b.lo = 0;
+  b.sm = lds_small + w * PB_SMALL;
+  b.bv = lds_big + w * 2 * PB_BIG;
+  b.bn = b.bv + PB_BIG;
+}
+
+// Called converged by the whole wavefront.
+__device__ void wp_flush(WavePush &b, const JobConst *jcs, JobState *jss, const GraphDev &g) {
+  __builtin_amdgcn_wave_barrier();
+  const int lane = lane_id();
+  if (b.jb >= 0 && (b.ns || b.nb)) {
+    const JobConst &c = jcs[b.jb];
+    JobState *s = &jss[b.jb];
+    if (b.ns) {
+      int base = 0;
+      if (lane == 0) base = atomicAdd(&s->cnt[b.lo][0], b.ns);
+      base = __shfl(base, 0, 64);
+      int32_t *dst = c.lst + (size_t)b.lo * g.n + base;
+      for (int i = lane; i < b.ns; i += WAVE) dst[i] = b.sm[i];
+    }
+    if (b.nb) {
+      const int nch = lane < b.nb ? b.bn[lane] : 0;
+      int x = nch;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+      }
+      const int tot = __shfl(x, 63, 64);
+      int base = 0;
+      if (lane == 0) base = atomicAdd(&s->cnt[b.lo][1], tot);
+      base = __shfl(base, 0, 64);
+      if (lane < b.nb) {
+        const int32_t v = b.bv[lane];
+        int32_t *dst = c.blst + ((size_t)b.lo * c.bcap + base + x - nch) * 2;
+        for (int i = 0; i < nch; i++) { dst[2 * i] = v; dst[2 * i + 1] = i; }
+      }
+    }
+  }
+  b.ns = b.nb = 0;
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Buffer u (for lanes with want) into job jb's list lo. Called converged.
+__device__ __forceinline__ void wp_push(WavePush &b, const JobConst *jcs, JobState *jss, const GraphDev &g,
+                                        int jb, int lo, bool want, int32_t u, int nch_u) {
+  const unsigned long long wm = __ballot(want);
+  if (!wm) return;
+  const int first = __builtin_ctzll(wm);
+  const int jbF = __shfl(jb, first, 64), loF = __shfl(lo, first, 64);
+  if (!__all(!want || (jb == jbF && lo == loF))) {  // mixed jobs in one step: rare
+    wp_flush(b, jcs, jss, g);
+    if (want) push_vertex(jcs[jb], &jss[jb], g, lo, u);
+    return;
+  }
+  if (b.jb != jbF || b.lo != loF) {
+    wp_flush(b, jcs, jss, g);
+    b.jb = jbF;
+    b.lo = loF;
+  }
+  const bool small = want && nch_u == 0;
+  const bool big = want && nch_u > 0;
+  const unsigned long long ms = __ballot(small), mb = __ballot(big);
+  const int cs = __popcll(ms), cb = __popcll(mb);
+  if (b.ns + cs > PB_SMALL || b.nb + cb > PB_BIG) wp_flush(b, jcs, jss, g);
+  const unsigned lo32s = (unsigned)ms, hi32s = (unsigned)(ms >> 32);
+  const unsigned lo32b = (unsigned)mb, hi32b = (unsigned)(mb >> 32);
+  if (small) b.sm[b.ns + __builtin_amdgcn_mbcnt_hi(hi32s, __builtin_amdgcn_mbcnt_lo(lo32s, 0))] = u;
+  if (big) {
+    const int r = b.nb + __builtin_amdgcn_mbcnt_hi(hi32b, __builtin_amdgcn_mbcnt_lo(lo32b, 0));
+    b.bv[r] = u;
+    b.bn[r] = nch_u;
+  }
+  b.ns += cs;
+  b.nb += cb;
+}
+
+// --------------------------------------------------------------------------- select
+// Non-hop: evaluate every frontier vertex's active states once per round (update UB,
+// apply the UB/2 and lower-bound cuts) and overwrite its mask with the states that grow
+// now. States above the near-far threshold move to the far pile. Hub vertices spread
+// over many wavefront items are evaluated once, not per item.
+__global__ __launch_bounds__(BLK) void k_select(const JobConst *jcs, JobState *jss, const Plan *pl, GraphDev g) {
+  __shared__ long long so0[MAXJOBS + 1], so1[MAXJOBS + 1];
+  __shared__ int32_t pb_small[(BLK / WAVE) * PB_SMALL];
+  __shared__ int32_t pb_big[(BLK / WAVE) * 2 * PB_BIG];
+  WavePush pb;
+  wp_init(pb, pb_small, pb_big);
+  const int nj = load_plan(pl, WK_THR, so0);
+  load_plan(pl, WK_WAVE, so1);
+  const long long t0 = so0[nj], total = t0 + so1[nj];
+  GrowCounters k;
+  k.zero();
+  int cj = -1;
+  const long long base0 = (long long)blockIdx.x * BLK + (threadIdx.x & ~63);
+  for (long long wbase = base0; wbase < total; wbase += (long long)gridDim.x * BLK) {
+    const long long it = wbase + lane_id();
+    int jb = -1, fl = 0;
+    int32_t v = 0;
+    bool want = false;
+    if (it < total) {
+      const int32_t *item;
+      if (it < t0) {
+        jb = find_job(so0, nj, it);
+        const JobConst &c = jcs[jb];
+        item = c.lst + (size_t)cur_list(c, &jss[jb]) * g.n + (it - so0[jb]);
+        v = item[0];
+      } else {
+        const long long i1 = it - t0;
+        jb = find_job(so1, nj, i1);
+        const JobConst &c = jcs[jb];
+        item = c.blst + ((size_t)cur_list(c, &jss[jb]) * c.bcap + (i1 - so1[jb])) * 2;
+        v = item[0];
+        if (item[1] != 0) jb = -1;  // one evaluation per vertex
+      }
+      if (jb >= 0 && jcs[jb].hop) jb = -1;
+      if (jb >= 0) {
+        if (jb != cj) { flush_lane(jss, cj, k); k.zero(); cj = jb; }
+        const JobConst &c = jcs[jb];
+        JobState *s = &jss[jb];
+        unsigned long long *act = c.act + (size_t)cur_list(c, s) * g.n;
+        const unsigned long long a = act[v];
+        const int32_t *row = c.T + (size_t)v * c.W;
+        unsigned long long defer = 0;
+        const unsigned long long gm =
+            grow_select(c, s, row, a, s->level, load_ub(s), k.states, s->theta, defer, k.dmin);
+        if (gm != a) act[v] = gm;
+        fl = FAR_LIST + s->fpar;
+        if (defer) want = atomicOr(&c.act[(size_t)FAR_ACT * g.n + v], defer) == 0;
+      }
+    }
+    wp_push(pb, jcs, jss, g, jb, fl, want, v, 0);
+  }
+  wp_flush(pb, jcs, jss, g);
+  flush_counters(jss, cj, k);
+}
+
+// Near-far: the near frontier is empty and the threshold was raised; move the far
+// states now under it to the next near list, the rest to the other far pile.
+__global__ __launch_bounds__(BLK) void k_farsplit(const JobConst *jcs, JobState *jss, const Plan *pl, GraphDev g) {
+  __shared__ long long soff[MAXJOBS + 1];
+  __shared__ int32_t pb_small[(BLK / WAVE) * PB_SMALL];
+  __shared__ int32_t pb_big[(BLK / WAVE) * 2 * PB_BIG];
+  WavePush pb;
+  wp_init(pb, pb_small, pb_big);
+  const int nj = load_plan(pl, WK_FAR, soff);
+  const long long total = soff[nj];
+  GrowCounters k;
+  k.zero();
+  int cj = -1;
+  const long long base0 = (long long)blockIdx.x * BLK + (threadIdx.x & ~63);
+  for (long long wbase = base0; wbase < total; wbase += (long long)gridDim.x * BLK) {
+    const long long it = wbase + lane_id();
+    int jb = -1, lo = 0, fl = 0, nch = 0;
+    int32_t v = 0;
+    bool wantN = false, wantF = false;
+    if (it < total) {
+      jb = find_job(soff, nj, it);
+      if (jb != cj) { flush_lane(jss, cj, k); k.zero(); cj = jb; }
+      const JobConst &c = jcs[jb];
+      JobState *s = &jss[jb];
+      unsigned long long *far = c.act + (size_t)FAR_ACT * g.n;
+      v = c.lst[(size_t)(FAR_LIST + s->fpar) * g.n + (it - soff[jb])];
+      const unsigned long long a = far[v];
+      far[v] = 0;
+      lo = s->parity ^ 1;
+      fl = FAR_LIST + (s->fpar ^ 1);
+      nch = g.nchunk[v];
+      unsigned long long defer = 0;
+      int dummy = 0;
+      const unsigned long long near = grow_select(c, s, c.T + (size_t)v * c.W, a, s->level, load_ub(s), dummy,
+                                                  s->theta, defer, k.dmin);
+      if (near) wantN = atomicOr(&c.act[(size_t)lo * g.n + v], near) == 0;
+      if (defer) wantF = atomicOr(&far[v], defer) == 0;
+    }
+    wp_push(pb, jcs, jss, g, jb, lo, wantN, v, nch);
+    wp_push(pb, jcs, jss, g, jb, fl, wantF, v, 0);
+  }
+  wp_flush(pb, jcs, jss, g);
+  flush_counters(jss, cj, k);
+}
+
+// --------------------------------------------------------------------------- grow
+// Relax up to two edges of one lane for the subsets of G. The target states, the
+// targets' next-round masks and chunk counts are all loaded in one batch; the arena
+// atomicMin is fire-and-forget; the mask atomicOr (whose result decides who pushes the
+// target) is skipped when the bits are already set.
+__device__ __forceinline__ void relax_pair(int32_t *T, size_t RW, size_t dsh, unsigned long long *act_lo,
+                                           const int32_t *nchunk, const RelaxGroup &G, bool h0, int32_t u0,
+                                           int32_t w0, bool h1, int32_t u1, int32_t w1, bool &want0,
+                                           bool &want1, int &n0, int &n1, int &n_atom) {
+  int32_t *Td0 = T + (size_t)u0 * RW + dsh, *Td1 = T + (size_t)u1 * RW + dsh;
+  int t0[RG], t1[RG];
+#pragma unroll
+  for (int i = 0; i < RG; i++) {
+    t0[i] = (h0 && i < G.n) ? Td0[G.p[i]] : 0;
+    t1[i] = (h1 && i < G.n) ? Td1[G.p[i]] : 0;
+  }
+  const unsigned long long cur0 = h0 ? act_lo[u0] : 0, cur1 = h1 ? act_lo[u1] : 0;
+  n0 = h0 ? nchunk[u0] : 0;
+  n1 = h1 ? nchunk[u1] : 0;
+  unsigned long long imp0 = 0, imp1 = 0;
+#pragma unroll
+  for (int i = 0; i < RG; i++) {
+    if (i < G.n) {
+      if (h0) {
+        int nc = G.c[i] + w0;
+        nc = nc < INF ? nc : INF;
+        if (nc < t0[i]) { atomicMin(&Td0[G.p[i]], nc); n_atom++; imp0 |= 1ull << G.j[i]; }
+      }
+      if (h1) {
+        int nd = G.c[i] + w1;
+        nd = nd < INF ? nd : INF;
+        if (nd < t1[i]) { atomicMin(&Td1[G.p[i]], nd); n_atom++; imp1 |= 1ull << G.j[i]; }
+      }
+    }
+  }
+  want0 = (imp0 & ~cur0) ? atomicOr(&act_lo[u0], imp0) == 0 : false;
+  want1 = (imp1 & ~cur1) ? atomicOr(&act_lo[u1], imp1) == 0 : false;
+}
+
+// Thread per vertex (degree < split). The edge loop is uniform across the wavefront
+// (lanes run (group, edge-pair) steps in lock step), so worklist pushes are aggregated.
+// Job fields are copied to registers: the arena atomics could otherwise alias them and
+// force a reload on every step.
 __global__ __launch_bounds__(BLK) void k_grow_thread(const JobConst *jcs, JobState *jss, const Plan *pl, GraphDev g) {
   __shared__ long long soff[MAXJOBS + 1];
+  __shared__ int32_t pb_small[(BLK / WAVE) * PB_SMALL];
+  __shared__ int32_t pb_big[(BLK / WAVE) * 2 * PB_BIG];
+  WavePush pb;
+  wp_init(pb, pb_small, pb_big);
   const int nj = load_plan(pl, WK_THR, soff);
   const long long total = soff[nj];
   GrowCounters k;
   k.zero();
   int cj = -1;
-  for (long long it = (long long)blockIdx.x * BLK + threadIdx.x; it < total; it += (long long)gridDim.x * BLK) {
-    const int jb = find_job(soff, nj, it);
-    if (jb != cj) { flush_lane(jss, cj, k); k.zero(); cj = jb; }
-    const JobConst &c = jcs[jb];
-    JobState *s = &jss[jb];
-    const int li = cur_list(c, s), lo = nxt_list(c, s);
-    const int L = s->level;
-    const int RW = c.H * c.W;
-    const int32_t v = c.lst[(size_t)li * g.n + (it - soff[jb])];
-    const unsigned long long a = c.act[(size_t)li * g.n + v];
-    k.verts++;
-    int32_t *src = c.T + (size_t)v * RW + (c.hop ? s->depth * c.W : 0);
-    unsigned long long gm;
-    if (c.hop) { gm = a; k.states += popc((uint32_t)a) + popc((uint32_t)(a >> 32)); }
-    else gm = grow_select(c, s, src, a, L, load_ub(s), k.states);
-    if (!gm) continue;
-    const int nb = popc((uint32_t)gm) + popc((uint32_t)(gm >> 32));
-    const int32_t e0 = g.beg[v], e1 = g.beg[v + 1];
-    k.edges += e1 - e0;
-    k.relax += (e1 - e0) * nb;
-    for (int32_t e = e0; e < e1; e++) {
-      const int32_t u = g.adj[e];
-      int32_t *dst = c.T + (size_t)u * RW + (c.hop ? (s->depth + 1) * c.W : 0);
-      const unsigned long long imp = relax_edge(src, dst, c, L, gm, g.w[e], k.atom);
-      if (imp) {
-        const unsigned long long old = atomicOr(&c.act[(size_t)lo * g.n + u], imp);
-        if (old == 0) { push_vertex(c, s, g, lo, u); k.push++; }
+  const long long stride = (long long)gridDim.x * BLK;
+  const int32_t *__restrict__ adj = g.adj;
+  const int32_t *__restrict__ wts = g.w;
+  const long long base0 = (long long)blockIdx.x * BLK + (threadIdx.x & ~63);
+  for (long long wbase = base0; wbase < total; wbase += stride) {
+    const long long it = wbase + lane_id();
+    const bool has = it < total;
+    int jb = -1, lo = 0, L = 0, deg = 0;
+    int32_t v = 0, e0 = 0;
+    int32_t *src = nullptr, *T = nullptr;
+    unsigned long long *act_lo = nullptr;
+    size_t RW = 0, dsh = 0;
+    unsigned long long gm = 0;
+    if (has) {
+      jb = find_job(soff, nj, it);
+      if (jb != cj) { flush_lane(jss, cj, k); k.zero(); cj = jb; }
+      const JobConst &c = jcs[jb];
+      JobState *s = &jss[jb];
+      const int li = cur_list(c, s);
+      lo = nxt_list(c, s);
+      L = s->level;
+      RW = (size_t)c.H * c.W;
+      T = c.T;
+      act_lo = c.act + (size_t)lo * g.n;
+      v = c.lst[(size_t)li * g.n + (it - soff[jb])];
+      gm = c.act[(size_t)li * g.n + v];  // already reduced by k_select (non-hop)
+      k.verts++;
+      if (c.hop) k.states += popc((uint32_t)gm) + popc((uint32_t)(gm >> 32));
+      src = T + (size_t)v * RW + (c.hop ? s->depth * c.W : 0);
+      dsh = c.hop ? (size_t)(s->depth + 1) * c.W : 0;
+      e0 = g.beg[v];
+      deg = g.beg[v + 1] - e0;
+      if (gm && deg) {
+        const int nb = popc((uint32_t)gm) + popc((uint32_t)(gm >> 32));
+        k.edges += deg;
+        k.relax += deg * nb;
+      } else {
+        gm = 0;
       }
     }
+    RelaxGroup G;
+    G.n = 0;
+    int ei = 0;
+    bool live = gm != 0;
+    if (live) take_group(jcs[jb], L, src, gm, G);
+    for (;;) {
+      if (!__any(live)) break;
+      bool want0 = false, want1 = false;
+      int32_t u0 = 0, u1 = 0;
+      int n0 = 0, n1 = 0;
+      if (live) {
+        const bool two = ei + 1 < deg;
+        u0 = adj[e0 + ei];
+        const int32_t w0 = wts[e0 + ei];
+        u1 = two ? adj[e0 + ei + 1] : u0;
+        const int32_t w1 = two ? wts[e0 + ei + 1] : w0;
+        relax_pair(T, RW, dsh, act_lo, g.nchunk, G, true, u0, w0, two, u1, w1, want0, want1, n0, n1, k.atom);
+        ei += two ? 2 : 1;
+        if (ei >= deg) {
+          ei = 0;
+          if (gm) take_group(jcs[jb], L, src, gm, G);
+          else live = false;
+        }
+      }
+      wp_push(pb, jcs, jss, g, jb, lo, want0, u0, n0);
+      wp_push(pb, jcs, jss, g, jb, lo, want1, u1, n1);
+      k.push += (int)want0 + (int)want1;
+    }
   }
+  wp_flush(pb, jcs, jss, g);
   flush_counters(jss, cj, k);
 }
 
-// Wavefront per (vertex, edge chunk) for vertices with degree >= split.
+// Wavefront per (vertex, edge chunk) for vertices with degree >= split. Each lane
+// relaxes two edges per strip of 128.
 __global__ __launch_bounds__(BLK) void k_grow_wave(const JobConst *jcs, JobState *jss, const Plan *pl, GraphDev g) {
   __shared__ long long soff[MAXJOBS + 1];
+  __shared__ int32_t pb_small[(BLK / WAVE) * PB_SMALL];
+  __shared__ int32_t pb_big[(BLK / WAVE) * 2 * PB_BIG];
+  WavePush pb;
+  wp_init(pb, pb_small, pb_big);
   const int nj = load_plan(pl, WK_WAVE, soff);
   const long long total = soff[nj];
-  const int lane = threadIdx.x & 63;
+  const int lane = lane_id();
   GrowCounters k;
   k.zero();
   int cj = -1;
+  const int32_t *__restrict__ adj = g.adj;
+  const int32_t *__restrict__ wts = g.w;
   const long long waves_per_grid = (long long)gridDim.x * (BLK / WAVE);
   for (long long it = (long long)blockIdx.x * (BLK / WAVE) + (threadIdx.x >> 6); it < total; it += waves_per_grid) {
     const int jb = find_job(soff, nj, it);
@@ -514,35 +880,49 @@ __global__ __launch_bounds__(BLK) void k_grow_wave(const JobConst *jcs, JobState
     JobState *s = &jss[jb];
     const int li = cur_list(c, s), lo = nxt_list(c, s);
     const int L = s->level;
-    const int RW = c.H * c.W;
+    const size_t RW = (size_t)c.H * c.W;
+    int32_t *const T = c.T;
+    unsigned long long *const act_lo = c.act + (size_t)lo * g.n;
     const int32_t *item = c.blst + ((size_t)li * c.bcap + (it - soff[jb])) * 2;
     const int32_t v = item[0], ch = item[1];
-    const unsigned long long a = c.act[(size_t)li * g.n + v];
-    int32_t *src = c.T + (size_t)v * RW + (c.hop ? s->depth * c.W : 0);
-    unsigned long long gm;
-    int nst = 0;
-    if (c.hop) gm = a;
-    else gm = grow_select(c, s, src, a, L, load_ub(s), nst);  // every lane computes the same mask
-    if (lane == 0) {
-      k.verts += ch == 0;
-      k.states += ch == 0 ? (c.hop ? popc((uint32_t)a) + popc((uint32_t)(a >> 32)) : nst) : 0;
+    unsigned long long gm = c.act[(size_t)li * g.n + v];  // already reduced by k_select (non-hop)
+    int32_t *src = T + (size_t)v * RW + (c.hop ? s->depth * c.W : 0);
+    const size_t dsh = c.hop ? (size_t)(s->depth + 1) * c.W : 0;
+    if (lane == 0 && ch == 0) {
+      k.verts++;
+      if (c.hop) k.states += popc((uint32_t)gm) + popc((uint32_t)(gm >> 32));
     }
     if (!gm) continue;
     const int nb = popc((uint32_t)gm) + popc((uint32_t)(gm >> 32));
     const int32_t eb = g.beg[v] + ch * g.chunk;
     const int32_t ee = min(g.beg[v + 1], eb + g.chunk);
-    for (int32_t e = eb + lane; e < ee; e += WAVE) {
-      const int32_t u = g.adj[e];
-      k.edges++;
-      k.relax += nb;
-      int32_t *dst = c.T + (size_t)u * RW + (c.hop ? (s->depth + 1) * c.W : 0);
-      const unsigned long long imp = relax_edge(src, dst, c, L, gm, g.w[e], k.atom);
-      if (imp) {
-        const unsigned long long old = atomicOr(&c.act[(size_t)lo * g.n + u], imp);
-        if (old == 0) { push_vertex(c, s, g, lo, u); k.push++; }
+    if (lane == 0) {
+      k.edges += ee - eb;
+      k.relax += (ee - eb) * nb;
+    }
+    while (gm) {
+      RelaxGroup G;
+      take_group(c, L, src, gm, G);
+      for (int32_t e = eb; e < ee; e += 2 * WAVE) {  // uniform trip count
+        const int32_t e0 = e + lane, e1 = e + WAVE + lane;
+        const bool h0 = e0 < ee, h1 = e1 < ee;
+        bool want0 = false, want1 = false;
+        int32_t u0 = 0, u1 = 0;
+        int n0 = 0, n1 = 0;
+        if (h0) {
+          u0 = adj[e0];
+          const int32_t w0 = wts[e0];
+          u1 = h1 ? adj[e1] : u0;
+          const int32_t w1 = h1 ? wts[e1] : w0;
+          relax_pair(T, RW, dsh, act_lo, g.nchunk, G, true, u0, w0, h1, u1, w1, want0, want1, n0, n1, k.atom);
+        }
+        wp_push(pb, jcs, jss, g, jb, lo, want0, u0, n0);
+        wp_push(pb, jcs, jss, g, jb, lo, want1, u1, n1);
+        k.push += (int)want0 + (int)want1;
       }
     }
   }
+  wp_flush(pb, jcs, jss, g);
   flush_counters(jss, cj, k);
 }
 
@@ -640,7 +1020,7 @@ __global__ __launch_bounds__(BLK) void k_merge(const JobConst *jcs, JobState *js
   __shared__ long long soff[MAXJOBS + 1];
   __shared__ int32_t rows[MERGE_LDS_WORDS];
   __shared__ unsigned long long amask[512];
-  __shared__ int ubs;
+  __shared__ int ubs, minc_s;
   const int nj = load_plan(pl, WK_MERGE, soff);
   const long long total = soff[nj];
   for (long long gt = blockIdx.x; gt < total; gt += gridDim.x) {
@@ -662,10 +1042,10 @@ __global__ __launch_bounds__(BLK) void k_merge(const JobConst *jcs, JobState *js
     const int tv = (int)min((long long)TV, (long long)g.n - v0);
     for (int i = threadIdx.x; i < tv * W; i += BLK) rows[i] = c.T[(size_t)v0 * W + i];
     for (int i = threadIdx.x; i < tv; i += BLK) amask[i] = 0;
-    if (threadIdx.x == 0) ubs = load_ub(s);
+    if (threadIdx.x == 0) { ubs = load_ub(s); minc_s = INF; }
     __syncthreads();
     const int ub = ubs;
-    int ubl = ub;
+    int ubl = ub, mact = INF;
     for (int it = threadIdx.x; it < tv * nL; it += BLK) {
       const int vi = it / nL, j = it - vi * nL;
       const int p = c.sub[L][j];
@@ -688,12 +1068,17 @@ __global__ __launch_bounds__(BLK) void k_merge(const JobConst *jcs, JobState *js
       if (L < K) {
         bool grow = true;
         if (prune) grow = m <= (ub >> 1) && m + lb_tail(row, r1, p, c, hk, w2) <= ub;
-        if (grow) atomicOr(&amask[vi], 1ull << j);
+        if (grow) {
+          atomicOr(&amask[vi], 1ull << j);
+          mact = m < mact ? m : mact;
+        }
       }
     }
     if (ubl < ub) atomicMin(&ubs, ubl);
+    if (mact < INF) atomicMin(&minc_s, mact);
     __syncthreads();
     if (threadIdx.x == 0 && ubs < ub) atomicMin(&s->ub, ubs);
+    if (threadIdx.x == 0 && minc_s < INF) atomicMin(&s->minc, minc_s);
     for (int vi = threadIdx.x; vi < tv; vi += BLK) {
       if (amask[vi]) {
         const int32_t v = (int32_t)(v0 + vi);

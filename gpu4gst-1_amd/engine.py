@@ -37,7 +37,7 @@ class GstGroups(ctypes.Structure):
 class GstOpts(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int32), ("degree_split", ctypes.c_int32), ("chunk_edges", ctypes.c_int32),
                 ("max_batch", ctypes.c_int32), ("dp_budget_bytes", ctypes.c_int64), ("flags", ctypes.c_int32),
-                ("graph_rounds", ctypes.c_int32)]
+                ("graph_rounds", ctypes.c_int32), ("near_far_delta", ctypes.c_int32)]
 
 
 class GstQuery(ctypes.Structure):
@@ -87,7 +87,7 @@ def lib():
         L.gst_poll.argtypes = [P, ctypes.c_uint64, ctypes.POINTER(GstResult)]
         L.gst_wait.argtypes = [P, ctypes.c_uint64, ctypes.POINTER(GstResult)]
         L.gst_info_get.argtypes = [P, ctypes.POINTER(GstInfo)]
-        L.gst_set_tuning.argtypes = [P, i32, i32]
+        L.gst_set_tuning.argtypes = [P, i32, i32, i32]
         L.gst_result_free.argtypes = [ctypes.POINTER(GstResult)]
         L.gst_close.argtypes = [P]
         L.gst_strerror.restype = ctypes.c_char_p
@@ -124,7 +124,7 @@ class Engine:
     """One resident graph + group table on one GPU."""
 
     def __init__(self, graph, groups, device=0, degree_split=0, chunk_edges=0, max_batch=0,
-                 dp_budget_bytes=0, flags=0, graph_rounds=0):
+                 dp_budget_bytes=0, flags=0, graph_rounds=0, near_far_delta=0):
         L = lib()
         self._beg = np.ascontiguousarray(graph.beg_pos, np.int64)
         self._adj = np.ascontiguousarray(graph.adj, np.int32)
@@ -134,7 +134,8 @@ class Engine:
         csr = GstCsr(len(self._beg) - 1, len(self._adj), self._beg.ctypes.data, self._adj.ctypes.data,
                      self._w.ctypes.data)
         grp = GstGroups(len(self._goffs) - 1, self._goffs.ctypes.data, self._gverts.ctypes.data)
-        opts = GstOpts(device, degree_split, chunk_edges, max_batch, dp_budget_bytes, flags, graph_rounds)
+        opts = GstOpts(device, degree_split, chunk_edges, max_batch, dp_budget_bytes, flags, graph_rounds,
+                       near_far_delta)
         h = ctypes.c_void_p()
         rc = L.gst_open(ctypes.byref(csr), ctypes.byref(grp), ctypes.byref(opts), ctypes.byref(h))
         if rc != GST_OK:
@@ -186,8 +187,8 @@ class Engine:
         d = self.solve(gids, D)
         return (d["cost"] if d["status"] == GST_OK else 100000), d
 
-    def set_tuning(self, degree_split, chunk_edges):
-        rc = lib().gst_set_tuning(self._h, int(degree_split), int(chunk_edges))
+    def set_tuning(self, degree_split, chunk_edges, near_far_delta=0):
+        rc = lib().gst_set_tuning(self._h, int(degree_split), int(chunk_edges), int(near_far_delta))
         if rc:
             raise GstError(rc, lib().gst_last_error().decode())
 

@@ -78,13 +78,16 @@ typedef struct {
 typedef struct {
   int32_t device;          /* HIP device ordinal */
   int32_t degree_split;    /* vertices with degree >= this are grown by a wavefront per
-                              edge chunk, the rest by a thread per vertex (0 = 32) */
-  int32_t chunk_edges;     /* edges per wavefront work item (0 = 512) */
+                              edge chunk, the rest by a thread per vertex (0 = 256) */
+  int32_t chunk_edges;     /* edges per wavefront work item (0 = 4096) */
   int32_t max_batch;       /* job slots of the batched/served forms (0 = 64) */
   int64_t dp_budget_bytes; /* DP memory budget of the batched/served forms
                               (0 = 75% of free device memory at gst_open) */
   int32_t flags;           /* default flags for every solve */
   int32_t graph_rounds;    /* rounds captured per HIP graph launch (0 = 16) */
+  int32_t near_far_delta;  /* near-far step of the non-hop grow frontier: states costlier
+                              than the threshold wait until the near frontier drains
+                              (<= 0 = off: plain label-correcting rounds) */
 } gst_opts;
 
 typedef struct {
@@ -134,8 +137,9 @@ int gst_submit(gst_ctx *ctx, const gst_query *queries, int32_t n, uint64_t *tick
 int gst_poll(gst_ctx *ctx, uint64_t ticket, gst_result *out);
 int gst_wait(gst_ctx *ctx, uint64_t ticket, gst_result *out);
 int gst_info_get(gst_ctx *ctx, gst_info *info);
-/* Tunables of the grow kernel (degree split, chunk size); used by the autotuner. */
-int gst_set_tuning(gst_ctx *ctx, int32_t degree_split, int32_t chunk_edges);
+/* Tunables of the grow phase (degree split, chunk size, near-far step, 0 = off); used by
+ * the autotuner. Call while no batched or served query is in flight. */
+int gst_set_tuning(gst_ctx *ctx, int32_t degree_split, int32_t chunk_edges, int32_t near_far_delta);
 void gst_result_free(gst_result *r);
 void gst_close(gst_ctx *ctx);
 const char *gst_strerror(int code);

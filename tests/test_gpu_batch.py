@@ -48,10 +48,13 @@ def test_batch_matches_plain_and_oracle(gpu_engine, rmat, rmat_inst):
 
 
 def test_batch_small_budget_serialises(rmat, rmat_inst):
-    # a budget that fits only one k=7 arena at a time forces queuing; results unchanged
-    budget = engine.arena_bytes(rmat.graph.n, 7) + 1
+    # a budget that fits the largest arena of the set, but never all of them at once,
+    # forces queuing; results unchanged
+    qs = _mixed_queries(rmat, 10, 2)
+    arenas = [engine.arena_bytes(rmat.graph.n, len(q.gids), q.D) for q in qs]
+    budget = max(arenas) + 1
+    assert sum(arenas) > budget
     with engine.Engine(rmat.graph, rmat.groups, dp_budget_bytes=budget, max_batch=8) as eng:
-        qs = _mixed_queries(rmat, 10, 2)
         res = eng.solve_batch(qs)
         for q, r in zip(qs, res):
             alone = eng.solve(q.gids, D=q.D)
@@ -84,7 +87,9 @@ def test_serving_continuous_admission(gpu_engine, rmat, rmat_inst):
 
 
 def test_serving_rejects_bad_and_oversized(rmat):
-    with engine.Engine(rmat.graph, rmat.groups, dp_budget_bytes=1 << 20) as eng:
+    budget = 1 << 16
+    assert engine.arena_bytes(rmat.graph.n, 3) > budget
+    with engine.Engine(rmat.graph, rmat.groups, dp_budget_bytes=budget) as eng:
         t = eng.submit([engine.Query(np.array([0, 1, 2], np.int32)),
                         engine.Query(np.array([rmat.groups.n_groups], np.int32))])
         r0 = eng.wait(t[0])
