@@ -1290,10 +1290,20 @@ __global__ __launch_bounds__(BLK) void k_rebuild(const JobConst *jcs, JobState *
     }
     const int pd = c.hop ? d - 1 : 0;
     const int32_t e0 = g.beg[v], e1 = g.beg[v + 1];
-    for (int32_t e = e0 + threadIdx.x; e < e1; e += BLK) {
-      const int32_t u = g.adj[e];
-      const long long t = (long long)c.T[(size_t)u * RW + pd * W + p] + g.w[e];
-      if (t <= cost) atomicMin((unsigned long long *)&found, (unsigned long long)(e - e0));
+    // ascending chunks of 4*BLK edges; the first chunk holding a predecessor has the
+    // smallest index, so the scan stops there (hubs have millions of edges)
+    for (int32_t cb = e0; cb < e1; cb += 4 * BLK) {
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int32_t e = cb + r * BLK + threadIdx.x;
+        if (e < e1) {
+          const int32_t u = g.adj[e];
+          const long long t = (long long)c.T[(size_t)u * RW + pd * W + p] + g.w[e];
+          if (t <= cost) atomicMin((unsigned long long *)&found, (unsigned long long)(e - e0));
+        }
+      }
+      __syncthreads();
+      if (found != 0x7fffffffffffffffll) break;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
