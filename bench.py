@@ -37,6 +37,68 @@ def b_grow(r):
             + 4 * r["grow_pushes"])
 
 
+def serve(args):
+    """Served form: Poisson arrivals at swept rates; p50/p99 latency from arrival to result;
+    every served result compared with the same query solved alone (plain path)."""
+    import torch
+
+    assert torch.cuda.is_available(), "bench needs a GPU"
+    t0 = time.time()
+    wl = datasets.make_workload(args.workload)
+    g = wl.graph
+    print(f"# serve workload {args.workload}: n={g.n} edges={g.n_edges} groups={wl.groups.n_groups} "
+          f"({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
+    rng = np.random.default_rng(datasets.SEED_BASE + 99)
+    total = args.serve_queries * len(args.rates)
+    qs = []
+    for _ in range(total):
+        k = int(rng.choice([3, 5, 7]))
+        D = int(rng.choice([-1, 3, 4, 5]))
+        qs.append(engine.Query(rng.choice(wl.groups.n_groups, k, replace=False).astype(np.int32), D))
+    eng = engine.Engine(g, wl.groups, max_batch=args.batch, dp_budget_bytes=int(args.budget_gb * 1e9))
+    # plain reference of every query (also warms the code paths)
+    alone = []
+    tp = time.perf_counter()
+    for q in qs:
+        alone.append(eng.solve(q.gids, D=q.D))
+    plain_dt = time.perf_counter() - tp
+    plain_qps = len(qs) / plain_dt
+    print(f"# plain: {plain_qps:.2f} q/s over {len(qs)} mixed queries ({time.time() - t0:.1f}s)",
+          file=sys.stderr, flush=True)
+    rows = []
+    same = 0
+    for ri, rate in enumerate(args.rates):
+        part = qs[ri * args.serve_queries:(ri + 1) * args.serve_queries]
+        ref = alone[ri * args.serve_queries:(ri + 1) * args.serve_queries]
+        gaps = rng.exponential(1.0 / rate, size=len(part))
+        tickets = []
+        start = time.perf_counter()
+        nxt = start
+        for q, gap in zip(part, gaps):
+            nxt += gap
+            while time.perf_counter() < nxt:
+                time.sleep(min(0.0005, max(0.0, nxt - time.perf_counter())))
+            tickets += eng.submit([q])
+        res = [eng.wait(t) for t in tickets]
+        dt = time.perf_counter() - start
+        lat = np.array([r["t_total_s"] for r in res]) * 1e3
+        same += sum(1 for r, a in zip(res, ref) if r["status"] == a["status"] and r["cost"] == a["cost"])
+        rows.append({"rate_qps": rate, "achieved_qps": round(len(res) / dt, 3), "p50_ms": round(float(np.median(lat)), 2),
+                     "p99_ms": round(float(np.percentile(lat, 99)), 2),
+                     "queue_p50_ms": round(float(np.median([r["t_queue_s"] for r in res])) * 1e3, 2)})
+        print(f"# rate {rate}: {rows[-1]} ({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
+    out = {"metric": "served GST queries/s and p50/p99 ms (Poisson arrivals, mixed k and D)",
+           "value": max(r["achieved_qps"] for r in rows), "unit": "queries/s", "n_gpus": 1,
+           "higher_is_better": True, "dtype": "int32", "data": "synthetic",
+           "config": {"workload": f"{args.workload}: n={g.n}, undirected edges={g.n_edges}, "
+                                  f"{wl.groups.n_groups} groups, k in {{3,5,7}}, D in {{none,3,4,5}}",
+                      "dp_budget_gb": args.budget_gb, "slots": args.batch},
+           "rates": rows, "plain_qps": round(plain_qps, 3),
+           "served_equal_alone": f"{same}/{len(qs)}"}
+    print(json.dumps(out))
+    eng.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -51,7 +113,15 @@ def main():
     ap.add_argument("--profile-only", action="store_true", help="plain path only (for rocprof runs)")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (library defaults)")
     ap.add_argument("--stats", action="store_true", help="count finite states (extra arena scan per query)")
+    ap.add_argument("--serve", action="store_true", help="served form with Poisson arrivals instead")
+    ap.add_argument("--rates", type=float, nargs="+", default=[2.0, 8.0, 32.0])
+    ap.add_argument("--serve-queries", type=int, default=32)
+    ap.add_argument("--budget-gb", type=float, default=200.0)
     args = ap.parse_args()
+    if args.serve and args.gpus == 1:
+        if args.workload == "cl3m":
+            args.workload = "cl4m"
+        return serve(args)
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
                                    "(the path shards by query, see the design notes)"}))

@@ -213,6 +213,12 @@ void fill_jobconst(const gst_ctx *c, const Slot &s, const Query &q, JobConst &jc
     jc.sub[L][cnt[L]++] = (uint8_t)p;
   }
   for (int L = 0; L <= q.K; L++) jc.nsub[L] = (uint8_t)cnt[L];
+  // level-major arena: level L (1..k) block of n * H * C(k,L) states
+  long long off = 0;
+  for (int L = 0; L <= 8; L++) {
+    jc.off[L] = off;
+    if (L >= 1 && L <= q.K) off += (long long)c->n * jc.H * cnt[L];
+  }
 }
 
 void init_jobstate(JobState &js) {
@@ -251,6 +257,10 @@ void finish_result(const gst_ctx *c, const Query &q, const JobState &js, const s
   out->grow_relax = (int64_t)js.st[ST_RELAX];
   out->grow_pushes = (int64_t)js.st[ST_PUSH];
   out->arena_bytes = arena_bytes_for(c->n, q.K, q.D);
+  for (int L = 0; L < 8; L++) {
+    out->level_edges[L] = (int64_t)js.stl[L][0];
+    out->level_relax[L] = (int64_t)js.stl[L][1];
+  }
   out->root = -1;
   out->root_height = -1;
   if (js.err) {

@@ -9,9 +9,13 @@
 // no host round trips).
 //
 // DP state T(v,p[,d]) = cost of the cheapest tree rooted at v that covers the query
-// groups in subset p (and has height d, hop variant). Arena layout: int32, vertex-major
-// rows, T[(v*H + d)*W + p] with W = 2^k, H = D+1 (H = 1 without a diameter bound), so a
-// vertex's states for one height sit in W consecutive words.
+// groups in subset p (and has height d, hop variant). Arena layout: int32, level-major:
+// level L = |p| is one block holding, for every (vertex, height) row, the C(k,L) subsets
+// of that size contiguously (ascending p, so bit j of a level mask is offset j):
+//   T[off[L] + ((v*H + d) * C(k,L)) + rank(p)],  H = D+1 (1 without a diameter bound).
+// A grow round of level L touches only that block (k=5: 20-40 B per vertex row), so a
+// relaxation reads one or two 64-B sectors and the level's working set is a fraction of
+// the arena.
 //
 // Non-hop schedule (level = popcount of p):
 //   level 1 : k-source SSSP = the singleton states (grow rounds)          [SSSP, a4]
@@ -79,6 +83,7 @@ struct JobConst {
   int32_t seed_total;         // members of the query groups (seed work items)
   int32_t tv;                 // vertices per merge tile
   int32_t delta;              // near-far step of the grow frontier (0 = off)
+  long long off[9];           // arena block offset of each level (subset size)
   int32_t gid[8];
   uint8_t nsub[8];
   uint8_t sub[8][64];         // sub[L][j]: j-th subset of popcount L, ascending
@@ -97,6 +102,7 @@ struct alignas(16) JobState {
   int32_t minc;               // min cost activated by the last merge (level start)
   int32_t pad2, pad3;
   unsigned long long st[8];
+  unsigned long long stl[8][2];  // per level: edges scanned, state-edge relaxations
 };
 
 // Non-hop list indices: near frontier 0/1 (parity), far pile 2/3 (fpar); mask arrays:
@@ -129,6 +135,16 @@ constexpr int JF_NO_TREE = 2;   // == GST_F_NO_TREE
 
 __device__ __forceinline__ int popc(uint32_t x) { return __builtin_popcount(x); }
 
+// Arena index of state (v, p, d).
+__device__ __forceinline__ size_t sidx(const JobConst &c, int32_t v, int p, int d) {
+  const int L = popc(p);
+  return (size_t)c.off[L] + ((size_t)v * c.H + d) * c.nsub[L] + c.rank[p];
+}
+// Row of (v, d) in level L's block: C(k,L) contiguous states.
+__device__ __forceinline__ int32_t *lrow(const JobConst &c, int L, int32_t v, int d) {
+  return c.T + c.off[L] + ((size_t)v * c.H + d) * c.nsub[L];
+}
+
 __device__ __forceinline__ int load_ub(const JobState *s) {
   return __hip_atomic_load(&s->ub, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -158,8 +174,8 @@ __device__ __forceinline__ void push_vertex(const JobConst &c, JobState *s, cons
 }
 
 // Admissible lower bound on the weight still needed to connect a tree rooted at v
-// (covering p) to the groups outside p. row points at v's non-hop row (W words).
-__device__ int lb_tail(const int32_t *row, int32_t r1, int p, const JobConst &c,
+// (covering p) to the groups outside p. sing[i] = T(v, {i}) (group distances).
+__device__ int lb_tail(const int32_t *sing, int32_t r1, int p, const JobConst &c,
                        const int32_t *hk, const int32_t *w2) {
   const int X = c.R ^ p;
   if (X == 0) return 0;
@@ -167,7 +183,7 @@ __device__ int lb_tail(const int32_t *row, int32_t r1, int p, const JobConst &c,
   int dl[MAXK];
 #pragma unroll
   for (int i = 0; i < MAXK; i++) {
-    int x = (i < K) ? row[1 << i] : INF;
+    int x = (i < K) ? sing[i] : INF;
     x = x < r1 ? x : r1;
     dl[i] = x < LB_CAP ? x : LB_CAP;
   }
@@ -197,22 +213,18 @@ __device__ int lb_tail(const int32_t *row, int32_t r1, int p, const JobConst &c,
 
 // Realizable upper bounds from state (v,p) with cost cst: tree + shortest paths to the
 // missing groups, and tree + complementary tree when that level is already final.
-__device__ __forceinline__ int ub_candidate(const int32_t *row, int p, int cst, int L,
-                                            const JobConst &c) {
+__device__ __forceinline__ int ub_candidate(const int32_t *sing, int p, int cst, int L,
+                                            const JobConst &c, int compv) {
   long long star = cst;
   bool ok = true;
   for (int i = 0; i < c.K; i++)
     if (!((p >> i) & 1)) {
-      const int x = row[1 << i];
+      const int x = sing[i];
       if (x >= INF) { ok = false; break; }
       star += x;
     }
   long long best = ok ? star : (long long)INF;
-  const int q = c.R ^ p;
-  if (q && popc(q) < L) {
-    const int x = row[q];
-    if (x < INF && (long long)cst + x < best) best = (long long)cst + x;
-  }
+  if (compv < INF && (long long)cst + compv < best) best = (long long)cst + compv;
   return best < INF ? (int)best : INF;
 }
 
@@ -388,7 +400,7 @@ __global__ __launch_bounds__(BLK) void k_seed(const JobConst *jcs, JobState *jss
     if (i >= c.K) continue;
     const int li = c.hop ? 0 : 1;
     const int32_t x = g.gverts[g.goffs[c.gid[i]] + loc];
-    c.T[(size_t)x * c.H * c.W + (1 << i)] = 0;
+    lrow(c, 1, x, 0)[i] = 0;
     const unsigned long long old = atomicOr(&c.act[(size_t)li * g.n + x], 1ull << i);
     if (old == 0) push_vertex(c, s, g, li, x);
   }
@@ -397,10 +409,13 @@ __global__ __launch_bounds__(BLK) void k_seed(const JobConst *jcs, JobState *jss
 // --------------------------------------------------------------------------- grow
 // Evaluate the active states of v (bits of a) at the current level: update UB, and
 // return the subset of bits that may grow. Non-hop only.
+// Non-hop: v's active states of level L are bits of a (bit j = offset j of v's level row).
 __device__ __forceinline__ unsigned long long grow_select(const JobConst &c, JobState *s,
-                                                          const int32_t *row, unsigned long long a,
+                                                          int32_t v, unsigned long long a,
                                                           int L, int ub, int &n_states, int theta,
                                                           unsigned long long &defer, int &dmin) {
+  const int32_t *row = lrow(c, L, v, 0);
+  const int32_t *sing = lrow(c, 1, v, 0);
   const int32_t *hk = c.tab;
   const int32_t *w2 = c.tab + c.K * c.K * c.W;
   const bool prune = !(c.flags & JF_NO_PRUNE);
@@ -411,9 +426,11 @@ __device__ __forceinline__ unsigned long long grow_select(const JobConst &c, Job
     a &= a - 1;
     n_states++;
     const int p = c.sub[L][j];
-    const int cst = row[p];
+    const int cst = row[j];
     if (cst >= INF) continue;
-    const int cand = ub_candidate(row, p, cst, L, c);
+    const int q = c.R ^ p;
+    const int compv = (q && popc(q) < L) ? c.T[sidx(c, v, q, 0)] : INF;
+    const int cand = ub_candidate(sing, p, cst, L, c, compv);
     if (cand < ubl) {
       const int cur = load_ub(s);
       if (cand < cur) atomicMin(&s->ub, cand);
@@ -421,7 +438,7 @@ __device__ __forceinline__ unsigned long long grow_select(const JobConst &c, Job
     }
     if (prune) {
       if (cst > (ubl >> 1)) continue;
-      if (L > 1 && cst + lb_tail(row, s->r1, p, c, hk, w2) > ubl) continue;
+      if (L > 1 && cst + lb_tail(sing, s->r1, p, c, hk, w2) > ubl) continue;
     }
     if (cst > theta) {  // near-far: wait in the far pile
       defer |= 1ull << j;
@@ -441,25 +458,33 @@ __device__ __forceinline__ unsigned long long grow_select(const JobConst &c, Job
 constexpr int RG = 8;
 struct RelaxGroup {
   int n;
-  int p[RG];   // subset ids
+  int p[RG];   // offsets in the level row (= bit of the level mask)
   int c[RG];   // source costs
-  int j[RG];   // bit within the level
 };
 
-__device__ __forceinline__ void take_group(const JobConst &c, int L, const int32_t *src,
-                                           unsigned long long &gm, RelaxGroup &G) {
+// src = the source vertex's level row
+__device__ __forceinline__ void take_group(const int32_t *src, unsigned long long &gm, RelaxGroup &G) {
   G.n = 0;
 #pragma unroll
   for (int i = 0; i < RG; i++) {
-    G.p[i] = 0; G.c[i] = INF; G.j[i] = 0;
+    G.p[i] = 0; G.c[i] = INF;
     if (gm) {
       const int j = __builtin_ctzll(gm);
       gm &= gm - 1;
-      G.p[i] = c.sub[L][j];
-      G.c[i] = src[G.p[i]];
-      G.j[i] = j;
+      G.p[i] = j;
+      G.c[i] = src[j];
       G.n = i + 1;
     }
+  }
+}
+
+// Make a group held identically by every lane live in scalar registers.
+__device__ __forceinline__ void uniform_group(RelaxGroup &G) {
+  G.n = __builtin_amdgcn_readfirstlane(G.n);
+#pragma unroll
+  for (int i = 0; i < RG; i++) {
+    G.p[i] = __builtin_amdgcn_readfirstlane(G.p[i]);
+    G.c[i] = __builtin_amdgcn_readfirstlane(G.c[i]);
   }
 }
 
@@ -474,6 +499,9 @@ struct GrowCounters {
 __device__ __forceinline__ void flush_lane(JobState *jss, int cj, const GrowCounters &k) {
   if (cj < 0) return;
   JobState *s = &jss[cj];
+  const int L = s->level & 7;
+  if (k.edges) atomicAdd(&s->stl[L][0], (unsigned long long)k.edges);
+  if (k.relax) atomicAdd(&s->stl[L][1], (unsigned long long)k.relax);
   if (k.verts) atomicAdd(&s->st[ST_VERTS], (unsigned long long)k.verts);
   if (k.edges) atomicAdd(&s->st[ST_EDGES], (unsigned long long)k.edges);
   if (k.states) atomicAdd(&s->st[ST_STATES], (unsigned long long)k.states);
@@ -503,6 +531,9 @@ __device__ void flush_counters(JobState *jss, int cj, GrowCounters &k) {
   }
   if ((threadIdx.x & 63) == 0) {
     JobState *s = &jss[first];
+    const int L = s->level & 7;
+    if (v[1]) atomicAdd(&s->stl[L][0], v[1]);
+    if (v[3]) atomicAdd(&s->stl[L][1], v[3]);
     const int map[6] = {ST_VERTS, ST_EDGES, ST_STATES, ST_RELAX, ST_PUSH, ST_ATOM};
 #pragma unroll
     for (int q = 0; q < 6; q++)
@@ -669,10 +700,9 @@ __global__ __launch_bounds__(BLK) void k_select(const JobConst *jcs, JobState *j
         JobState *s = &jss[jb];
         unsigned long long *act = c.act + (size_t)cur_list(c, s) * g.n;
         const unsigned long long a = act[v];
-        const int32_t *row = c.T + (size_t)v * c.W;
         unsigned long long defer = 0;
         const unsigned long long gm =
-            grow_select(c, s, row, a, s->level, load_ub(s), k.states, s->theta, defer, k.dmin);
+            grow_select(c, s, v, a, s->level, load_ub(s), k.states, s->theta, defer, k.dmin);
         if (gm != a) act[v] = gm;
         fl = FAR_LIST + s->fpar;
         if (defer) want = atomicOr(&c.act[(size_t)FAR_ACT * g.n + v], defer) == 0;
@@ -717,7 +747,7 @@ __global__ __launch_bounds__(BLK) void k_farsplit(const JobConst *jcs, JobState 
       nch = g.nchunk[v];
       unsigned long long defer = 0;
       int dummy = 0;
-      const unsigned long long near = grow_select(c, s, c.T + (size_t)v * c.W, a, s->level, load_ub(s), dummy,
+      const unsigned long long near = grow_select(c, s, v, a, s->level, load_ub(s), dummy,
                                                   s->theta, defer, k.dmin);
       if (near) wantN = atomicOr(&c.act[(size_t)lo * g.n + v], near) == 0;
       if (defer) wantF = atomicOr(&far[v], defer) == 0;
@@ -730,24 +760,22 @@ __global__ __launch_bounds__(BLK) void k_farsplit(const JobConst *jcs, JobState 
 }
 
 // --------------------------------------------------------------------------- grow
-// Relax up to two edges of one lane for the subsets of G. The target states, the
-// targets' next-round masks and chunk counts are all loaded in one batch; the arena
-// atomicMin is fire-and-forget; the mask atomicOr (whose result decides who pushes the
-// target) is skipped when the bits are already set.
-__device__ __forceinline__ void relax_pair(int32_t *T, size_t RW, size_t dsh, unsigned long long *act_lo,
+// Relax up to two edges of one lane for the subsets of G. The target states of both
+// edges are loaded in one batch; the arena atomicMin is fire-and-forget; the mask
+// atomicOr (whose result decides who pushes the target) is skipped when the bits are
+// already set.
+// dbase/dstride: target u's level row is dbase + u * dstride.
+__device__ __forceinline__ void relax_pair(int32_t *dbase, size_t dstride, unsigned long long *act_lo,
                                            const int32_t *nchunk, const RelaxGroup &G, bool h0, int32_t u0,
                                            int32_t w0, bool h1, int32_t u1, int32_t w1, bool &want0,
                                            bool &want1, int &n0, int &n1, int &n_atom) {
-  int32_t *Td0 = T + (size_t)u0 * RW + dsh, *Td1 = T + (size_t)u1 * RW + dsh;
+  int32_t *Td0 = dbase + (size_t)u0 * dstride, *Td1 = dbase + (size_t)u1 * dstride;
   int t0[RG], t1[RG];
 #pragma unroll
   for (int i = 0; i < RG; i++) {
     t0[i] = (h0 && i < G.n) ? Td0[G.p[i]] : 0;
     t1[i] = (h1 && i < G.n) ? Td1[G.p[i]] : 0;
   }
-  const unsigned long long cur0 = h0 ? act_lo[u0] : 0, cur1 = h1 ? act_lo[u1] : 0;
-  n0 = h0 ? nchunk[u0] : 0;
-  n1 = h1 ? nchunk[u1] : 0;
   unsigned long long imp0 = 0, imp1 = 0;
 #pragma unroll
   for (int i = 0; i < RG; i++) {
@@ -755,17 +783,22 @@ __device__ __forceinline__ void relax_pair(int32_t *T, size_t RW, size_t dsh, un
       if (h0) {
         int nc = G.c[i] + w0;
         nc = nc < INF ? nc : INF;
-        if (nc < t0[i]) { atomicMin(&Td0[G.p[i]], nc); n_atom++; imp0 |= 1ull << G.j[i]; }
+        if (nc < t0[i]) { atomicMin(&Td0[G.p[i]], nc); n_atom++; imp0 |= 1ull << G.p[i]; }
       }
       if (h1) {
         int nd = G.c[i] + w1;
         nd = nd < INF ? nd : INF;
-        if (nd < t1[i]) { atomicMin(&Td1[G.p[i]], nd); n_atom++; imp1 |= 1ull << G.j[i]; }
+        if (nd < t1[i]) { atomicMin(&Td1[G.p[i]], nd); n_atom++; imp1 |= 1ull << G.p[i]; }
       }
     }
   }
+  // masks and chunk counts are read only for targets that improved: every extra
+  // per-edge load would fetch another random 64-B sector
+  const unsigned long long cur0 = imp0 ? act_lo[u0] : ~0ull, cur1 = imp1 ? act_lo[u1] : ~0ull;
   want0 = (imp0 & ~cur0) ? atomicOr(&act_lo[u0], imp0) == 0 : false;
   want1 = (imp1 & ~cur1) ? atomicOr(&act_lo[u1], imp1) == 0 : false;
+  n0 = want0 ? nchunk[u0] : 0;
+  n1 = want1 ? nchunk[u1] : 0;
 }
 
 // Thread per vertex (degree < split). The edge loop is uniform across the wavefront
@@ -792,9 +825,9 @@ __global__ __launch_bounds__(BLK) void k_grow_thread(const JobConst *jcs, JobSta
     const bool has = it < total;
     int jb = -1, lo = 0, L = 0, deg = 0;
     int32_t v = 0, e0 = 0;
-    int32_t *src = nullptr, *T = nullptr;
+    int32_t *src = nullptr, *dbase = nullptr;
     unsigned long long *act_lo = nullptr;
-    size_t RW = 0, dsh = 0;
+    size_t dstride = 0;
     unsigned long long gm = 0;
     if (has) {
       jb = find_job(soff, nj, it);
@@ -804,15 +837,15 @@ __global__ __launch_bounds__(BLK) void k_grow_thread(const JobConst *jcs, JobSta
       const int li = cur_list(c, s);
       lo = nxt_list(c, s);
       L = s->level;
-      RW = (size_t)c.H * c.W;
-      T = c.T;
+      const int d = c.hop ? s->depth : 0;
+      dstride = (size_t)c.H * c.nsub[L];
+      dbase = lrow(c, L, 0, c.hop ? d + 1 : 0);
       act_lo = c.act + (size_t)lo * g.n;
       v = c.lst[(size_t)li * g.n + (it - soff[jb])];
       gm = c.act[(size_t)li * g.n + v];  // already reduced by k_select (non-hop)
       k.verts++;
       if (c.hop) k.states += popc((uint32_t)gm) + popc((uint32_t)(gm >> 32));
-      src = T + (size_t)v * RW + (c.hop ? s->depth * c.W : 0);
-      dsh = c.hop ? (size_t)(s->depth + 1) * c.W : 0;
+      src = lrow(c, L, v, d);
       e0 = g.beg[v];
       deg = g.beg[v + 1] - e0;
       if (gm && deg) {
@@ -827,7 +860,7 @@ __global__ __launch_bounds__(BLK) void k_grow_thread(const JobConst *jcs, JobSta
     G.n = 0;
     int ei = 0;
     bool live = gm != 0;
-    if (live) take_group(jcs[jb], L, src, gm, G);
+    if (live) take_group(src, gm, G);
     for (;;) {
       if (!__any(live)) break;
       bool want0 = false, want1 = false;
@@ -839,11 +872,11 @@ __global__ __launch_bounds__(BLK) void k_grow_thread(const JobConst *jcs, JobSta
         const int32_t w0 = wts[e0 + ei];
         u1 = two ? adj[e0 + ei + 1] : u0;
         const int32_t w1 = two ? wts[e0 + ei + 1] : w0;
-        relax_pair(T, RW, dsh, act_lo, g.nchunk, G, true, u0, w0, two, u1, w1, want0, want1, n0, n1, k.atom);
+        relax_pair(dbase, dstride, act_lo, g.nchunk, G, true, u0, w0, two, u1, w1, want0, want1, n0, n1, k.atom);
         ei += two ? 2 : 1;
         if (ei >= deg) {
           ei = 0;
-          if (gm) take_group(jcs[jb], L, src, gm, G);
+          if (gm) take_group(src, gm, G);
           else live = false;
         }
       }
@@ -873,21 +906,22 @@ __global__ __launch_bounds__(BLK) void k_grow_wave(const JobConst *jcs, JobState
   const int32_t *__restrict__ adj = g.adj;
   const int32_t *__restrict__ wts = g.w;
   const long long waves_per_grid = (long long)gridDim.x * (BLK / WAVE);
-  for (long long it = (long long)blockIdx.x * (BLK / WAVE) + (threadIdx.x >> 6); it < total; it += waves_per_grid) {
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: scalar registers
+  for (long long it = (long long)blockIdx.x * (BLK / WAVE) + wave; it < total; it += waves_per_grid) {
     const int jb = find_job(soff, nj, it);
     if (jb != cj) { flush_lane(jss, cj, k); k.zero(); cj = jb; }
     const JobConst &c = jcs[jb];
     JobState *s = &jss[jb];
     const int li = cur_list(c, s), lo = nxt_list(c, s);
     const int L = s->level;
-    const size_t RW = (size_t)c.H * c.W;
-    int32_t *const T = c.T;
+    const int d = c.hop ? s->depth : 0;
+    const size_t dstride = (size_t)c.H * c.nsub[L];
+    int32_t *const dbase = lrow(c, L, 0, c.hop ? d + 1 : 0);
     unsigned long long *const act_lo = c.act + (size_t)lo * g.n;
     const int32_t *item = c.blst + ((size_t)li * c.bcap + (it - soff[jb])) * 2;
     const int32_t v = item[0], ch = item[1];
     unsigned long long gm = c.act[(size_t)li * g.n + v];  // already reduced by k_select (non-hop)
-    int32_t *src = T + (size_t)v * RW + (c.hop ? s->depth * c.W : 0);
-    const size_t dsh = c.hop ? (size_t)(s->depth + 1) * c.W : 0;
+    int32_t *src = lrow(c, L, v, d);
     if (lane == 0 && ch == 0) {
       k.verts++;
       if (c.hop) k.states += popc((uint32_t)gm) + popc((uint32_t)(gm >> 32));
@@ -902,7 +936,8 @@ __global__ __launch_bounds__(BLK) void k_grow_wave(const JobConst *jcs, JobState
     }
     while (gm) {
       RelaxGroup G;
-      take_group(c, L, src, gm, G);
+      take_group(src, gm, G);
+      uniform_group(G);
       for (int32_t e = eb; e < ee; e += 2 * WAVE) {  // uniform trip count
         const int32_t e0 = e + lane, e1 = e + WAVE + lane;
         const bool h0 = e0 < ee, h1 = e1 < ee;
@@ -914,7 +949,7 @@ __global__ __launch_bounds__(BLK) void k_grow_wave(const JobConst *jcs, JobState
           const int32_t w0 = wts[e0];
           u1 = h1 ? adj[e1] : u0;
           const int32_t w1 = h1 ? wts[e1] : w0;
-          relax_pair(T, RW, dsh, act_lo, g.nchunk, G, true, u0, w0, h1, u1, w1, want0, want1, n0, n1, k.atom);
+          relax_pair(dbase, dstride, act_lo, g.nchunk, G, true, u0, w0, h1, u1, w1, want0, want1, n0, n1, k.atom);
         }
         wp_push(pb, jcs, jss, g, jb, lo, want0, u0, n0);
         wp_push(pb, jcs, jss, g, jb, lo, want1, u1, n1);
@@ -970,9 +1005,9 @@ __global__ __launch_bounds__(BLK) void k_tables(const JobConst *jcs, JobState *j
     const long long b0 = g.goffs[c.gid[a]], b1 = g.goffs[c.gid[a] + 1];
     for (long long t = b0 + threadIdx.x; t < b1; t += BLK) {
       const int32_t x = g.gverts[t];
-      const int32_t *row = c.T + (size_t)x * W;
+      const int32_t *sing = lrow(c, 1, x, 0);
       for (int b = 0; b < K; b++) {
-        int d = row[1 << b];
+        int d = sing[b];
         d = d < r1 ? d : r1;
         atomicMin(&vv[a * MAXK + b], d);
       }
@@ -1040,9 +1075,19 @@ __global__ __launch_bounds__(BLK) void k_merge(const JobConst *jcs, JobState *js
   {
     const long long v0 = (gt - soff[jb]) * TV;
     const int tv = (int)min((long long)TV, (long long)g.n - v0);
-    for (int i = threadIdx.x; i < tv * W; i += BLK) rows[i] = c.T[(size_t)v0 * W + i];
+    // stage the lower levels of the tile's rows into LDS, indexed by subset p
+    for (int i = threadIdx.x; i < tv * W; i += BLK) rows[i] = INF;
     for (int i = threadIdx.x; i < tv; i += BLK) amask[i] = 0;
     if (threadIdx.x == 0) { ubs = load_ub(s); minc_s = INF; }
+    __syncthreads();
+    for (int l = 1; l < L; l++) {
+      const int ns = c.nsub[l];
+      const int32_t *blk = c.T + c.off[l] + (size_t)v0 * ns;  // contiguous tile segment
+      for (int i = threadIdx.x; i < tv * ns; i += BLK) {
+        const int vi = i / ns, r = i - vi * ns;
+        rows[vi * W + c.sub[l][r]] = blk[i];
+      }
+    }
     __syncthreads();
     const int ub = ubs;
     int ubl = ub, mact = INF;
@@ -1062,12 +1107,16 @@ __global__ __launch_bounds__(BLK) void k_merge(const JobConst *jcs, JobState *js
       }
       if (m >= INF) continue;
       merged++;
-      c.T[(size_t)(v0 + vi) * W + p] = m;
-      const int cand = (L == K) ? m : ub_candidate(row, p, m, L, c);
+      c.T[(size_t)c.off[L] + (size_t)(v0 + vi) * nL + j] = m;
+      int sl[MAXK];
+#pragma unroll
+      for (int i = 0; i < MAXK; i++) sl[i] = (i < K) ? row[1 << i] : INF;
+      const int q = c.R ^ p;
+      const int cand = (L == K) ? m : ub_candidate(sl, p, m, L, c, (q && popc(q) < L) ? row[q] : INF);
       if (cand < ubl) ubl = cand;
       if (L < K) {
         bool grow = true;
-        if (prune) grow = m <= (ub >> 1) && m + lb_tail(row, r1, p, c, hk, w2) <= ub;
+        if (prune) grow = m <= (ub >> 1) && m + lb_tail(sl, r1, p, c, hk, w2) <= ub;
         if (grow) {
           atomicOr(&amask[vi], 1ull << j);
           mact = m < mact ? m : mact;
@@ -1113,8 +1162,18 @@ __global__ __launch_bounds__(BLK) void k_merge_hop(const JobConst *jcs, JobState
   {
     const long long v0 = (gt - soff[jb]) * TV;
     const int tv = (int)min((long long)TV, (long long)g.n - v0);
-    for (int i = threadIdx.x; i < tv * RW; i += BLK) rows[i] = c.T[(size_t)v0 * RW + i];
+    for (int i = threadIdx.x; i < tv * RW; i += BLK) rows[i] = INF;
     for (int i = threadIdx.x; i < tv * H; i += BLK) amask[i] = 0;
+    __syncthreads();
+    for (int l = 1; l < L; l++) {
+      const int ns = c.nsub[l];
+      const int32_t *blk = c.T + c.off[l] + (size_t)v0 * H * ns;
+      for (int i = threadIdx.x; i < tv * H * ns; i += BLK) {
+        const int vi = i / (H * ns), rem = i - vi * H * ns;
+        const int d = rem / ns, r = rem - d * ns;
+        rows[vi * RW + d * W + c.sub[l][r]] = blk[i];
+      }
+    }
     __syncthreads();
     for (int it = threadIdx.x; it < tv * nL * H; it += BLK) {
       const int vi = it / (nL * H), rem = it - vi * nL * H;
@@ -1143,7 +1202,7 @@ __global__ __launch_bounds__(BLK) void k_merge_hop(const JobConst *jcs, JobState
       }
       if (m >= INF) continue;
       merged++;
-      c.T[(size_t)(v0 + vi) * RW + nd * W + p] = m;
+      c.T[(size_t)c.off[L] + ((size_t)(v0 + vi) * H + nd) * nL + j] = m;
       if (L < K && nd < D) atomicOr(&amask[vi * H + nd], 1ull << j);
     }
     __syncthreads();
@@ -1180,7 +1239,7 @@ __global__ __launch_bounds__(BLK) void k_final(const JobConst *jcs, JobState *js
     const long long r0 = (gt - soff[jb]) * FINAL_CHUNK;
     const long long r1 = min(rows, r0 + FINAL_CHUNK);
     for (long long i = r0 + threadIdx.x; i < r1; i += BLK) {
-      const int32_t cst = c.T[(size_t)i * W + R];
+      const int32_t cst = c.T[(size_t)c.off[c.K] + i];  // level k holds one state per row
       if (cst < INF) {
         const unsigned long long key = ((unsigned long long)cst << 32) | (unsigned long long)i;
         best = key < best ? key : best;
@@ -1236,7 +1295,7 @@ __global__ __launch_bounds__(BLK) void k_rebuild(const JobConst *jcs, JobState *
     if (threadIdx.x == 0) {
       top--;
       cv = c.stk[3 * top]; cp = c.stk[3 * top + 1]; cd = c.stk[3 * top + 2];
-      ccost = c.T[(size_t)cv * RW + cd * W + cp];
+      ccost = c.T[sidx(c, cv, cp, cd)];
       found = 0x7fffffffffffffffll;
     }
     __syncthreads();
@@ -1261,7 +1320,7 @@ __global__ __launch_bounds__(BLK) void k_rebuild(const JobConst *jcs, JobState *
       if (c.hop) {
         if ((d1 > d2 ? d1 : d2) != d || d1 + d2 > D) continue;
       } else if (d1 | d2) continue;
-      const long long t = (long long)c.T[(size_t)v * RW + d1 * W + p1] + c.T[(size_t)v * RW + d2 * W + (p ^ p1)];
+      const long long t = (long long)c.T[sidx(c, v, p1, d1)] + c.T[sidx(c, v, p ^ p1, d2)];
       if (t <= cost) atomicMin((unsigned long long *)&found, (unsigned long long)it);
     }
     __syncthreads();
@@ -1298,7 +1357,7 @@ __global__ __launch_bounds__(BLK) void k_rebuild(const JobConst *jcs, JobState *
         const int32_t e = cb + r * BLK + threadIdx.x;
         if (e < e1) {
           const int32_t u = g.adj[e];
-          const long long t = (long long)c.T[(size_t)u * RW + pd * W + p] + g.w[e];
+          const long long t = (long long)c.T[sidx(c, u, p, pd)] + g.w[e];
           if (t <= cost) atomicMin((unsigned long long *)&found, (unsigned long long)(e - e0));
         }
       }

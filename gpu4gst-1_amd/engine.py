@@ -53,7 +53,8 @@ class GstResult(ctypes.Structure):
                 ("t_merge_s", ctypes.c_double), ("states_finite", ctypes.c_int64),
                 ("states_processed", ctypes.c_int64), ("grow_vertices", ctypes.c_int64),
                 ("grow_edges", ctypes.c_int64), ("grow_relax", ctypes.c_int64), ("grow_pushes", ctypes.c_int64),
-                ("arena_bytes", ctypes.c_int64)]
+                ("arena_bytes", ctypes.c_int64), ("level_edges", ctypes.c_int64 * 8),
+                ("level_relax", ctypes.c_int64 * 8)]
 
 
 class GstInfo(ctypes.Structure):
@@ -106,7 +107,9 @@ class GstError(RuntimeError):
 
 
 def _result_dict(r: GstResult) -> dict:
-    d = {f: getattr(r, f) for f, _ in GstResult._fields_ if f != "edges"}
+    d = {f: getattr(r, f) for f, _ in GstResult._fields_ if f not in ("edges", "level_edges", "level_relax")}
+    d["level_edges"] = list(r.level_edges)
+    d["level_relax"] = list(r.level_relax)
     n = max(r.n_edges, 0)
     d["edges"] = np.ctypeslib.as_array(r.edges, (3 * n,)).reshape(n, 3).copy() if n and r.edges else \
         np.zeros((0, 3), np.int32)

@@ -115,6 +115,8 @@ typedef struct {
   int64_t states_processed; /* frontier states evaluated (the CSV "process_num") */
   int64_t grow_vertices, grow_edges, grow_relax, grow_pushes; /* grow-kernel work counters */
   int64_t arena_bytes;    /* DP arena size V*2^k*(D+1)*4 */
+  int64_t level_edges[8]; /* grow work per DP level (subset size 1..k): edges scanned */
+  int64_t level_relax[8]; /*   and state-edge relaxations */
 } gst_result;
 
 typedef struct {
