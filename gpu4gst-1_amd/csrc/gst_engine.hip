@@ -204,7 +204,7 @@ void fill_jobconst(const gst_ctx *c, const Slot &s, const Query &q, JobConst &jc
   long long members = 0;
   for (int i = 0; i < q.K; i++) members += c->h_goffs[q.gids[i] + 1] - c->h_goffs[q.gids[i]];
   jc.seed_total = (int32_t)members;
-  jc.tv = jc.hop ? std::max(1, std::min(256, HMERGE_LDS_WORDS / (jc.H * jc.W)))
+  jc.tv = jc.hop ? std::max(1, std::min({256, HMERGE_LDS_WORDS / (jc.H * jc.W), HMERGE_MASKS / jc.H}))
                  : std::min(512, MERGE_LDS_WORDS / jc.W);
   int cnt[8] = {0};
   for (int p = 0; p < jc.W; p++) {

@@ -655,6 +655,49 @@ __device__ __forceinline__ void wp_push(WavePush &b, const JobConst *jcs, JobSta
   b.nb += cb;
 }
 
+// push_vertex for every lane with want, one counter atomic per (list, kind) present in
+// the wavefront instead of one per vertex. li may differ between lanes. Called converged.
+__device__ void push_vertex_wave(const JobConst &c, JobState *s, const GraphDev &g, int li,
+                                 bool want, int32_t u) {
+  const int lane = lane_id();
+  int nch = 0;
+  if (want) {
+    const int32_t d = g.beg[u + 1] - g.beg[u];
+    nch = d < g.split ? 0 : (d + g.chunk - 1) / g.chunk;
+  }
+  unsigned long long pend = __ballot(want);
+  while (pend) {
+    const int leader = __builtin_ctzll(pend);
+    const int l0 = __shfl(li, leader, 64);
+    const bool mine = ((pend >> lane) & 1) && li == l0;
+    const unsigned long long ms = __ballot(mine && nch == 0), mb = __ballot(mine && nch > 0);
+    if (ms) {
+      int base = 0;
+      if (lane == leader) base = atomicAdd(&s->cnt[l0][0], (int)__popcll(ms));
+      base = __shfl(base, leader, 64);
+      const int r = __builtin_amdgcn_mbcnt_hi((unsigned)(ms >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)ms, 0));
+      if (mine && nch == 0) c.lst[(size_t)l0 * g.n + base + r] = u;
+    }
+    if (mb) {
+      int x = (mine && nch > 0) ? nch : 0;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+      }
+      const int tot = __shfl(x, 63, 64);
+      int base = 0;
+      if (lane == leader) base = atomicAdd(&s->cnt[l0][1], tot);
+      base = __shfl(base, leader, 64);
+      if (mine && nch > 0) {
+        int32_t *b = c.blst + ((size_t)l0 * c.bcap + base + x - nch) * 2;
+        for (int i = 0; i < nch; i++) { b[2 * i] = u; b[2 * i + 1] = i; }
+      }
+    }
+    pend &= ~(ms | mb);
+  }
+}
+
 // --------------------------------------------------------------------------- select
 // Non-hop: evaluate every frontier vertex's active states once per round (update UB,
 // apply the UB/2 and lower-bound cuts) and overwrite its mask with the states that grow
@@ -1047,6 +1090,27 @@ __global__ __launch_bounds__(BLK) void k_tables(const JobConst *jcs, JobState *j
 }
 
 // --------------------------------------------------------------------------- merge
+// Copies n consecutive words of src into LDS through put(i, value). Each lane issues
+// SU loads before its first LDS store, so a tile's rows arrive after a few memory
+// latencies rather than one latency per word.
+constexpr int SU = 8;
+template <class Put>
+__device__ __forceinline__ void stage_block(const int32_t *__restrict__ src, int n, Put put) {
+  for (int base = 0; base < n; base += SU * BLK) {
+    int32_t x[SU];
+#pragma unroll
+    for (int u = 0; u < SU; u++) {
+      const int i = base + u * BLK + (int)threadIdx.x;
+      x[u] = i < n ? src[i] : INF;
+    }
+#pragma unroll
+    for (int u = 0; u < SU; u++) {
+      const int i = base + u * BLK + (int)threadIdx.x;
+      if (i < n) put(i, x[u]);
+    }
+  }
+}
+
 // Non-hop merge for level L: T(v,p) = min over splits p = p1 + p2 of T(v,p1) + T(v,p2).
 // A tile of vertices has its rows staged in LDS (coalesced contiguous read), each
 // thread evaluates (vertex, subset) items, new states are activated for growth.
@@ -1076,17 +1140,16 @@ __global__ __launch_bounds__(BLK) void k_merge(const JobConst *jcs, JobState *js
     const long long v0 = (gt - soff[jb]) * TV;
     const int tv = (int)min((long long)TV, (long long)g.n - v0);
     // stage the lower levels of the tile's rows into LDS, indexed by subset p
-    for (int i = threadIdx.x; i < tv * W; i += BLK) rows[i] = INF;
+    // every subset read below has fewer than L members, so staging overwrites all of it
     for (int i = threadIdx.x; i < tv; i += BLK) amask[i] = 0;
     if (threadIdx.x == 0) { ubs = load_ub(s); minc_s = INF; }
-    __syncthreads();
     for (int l = 1; l < L; l++) {
       const int ns = c.nsub[l];
-      const int32_t *blk = c.T + c.off[l] + (size_t)v0 * ns;  // contiguous tile segment
-      for (int i = threadIdx.x; i < tv * ns; i += BLK) {
+      const uint8_t *sub = c.sub[l];
+      stage_block(c.T + c.off[l] + (size_t)v0 * ns, tv * ns, [&](int i, int32_t x) {
         const int vi = i / ns, r = i - vi * ns;
-        rows[vi * W + c.sub[l][r]] = blk[i];
-      }
+        rows[vi * W + sub[r]] = x;
+      });
     }
     __syncthreads();
     const int ub = ubs;
@@ -1128,12 +1191,12 @@ __global__ __launch_bounds__(BLK) void k_merge(const JobConst *jcs, JobState *js
     __syncthreads();
     if (threadIdx.x == 0 && ubs < ub) atomicMin(&s->ub, ubs);
     if (threadIdx.x == 0 && minc_s < INF) atomicMin(&s->minc, minc_s);
-    for (int vi = threadIdx.x; vi < tv; vi += BLK) {
-      if (amask[vi]) {
-        const int32_t v = (int32_t)(v0 + vi);
-        c.act[(size_t)lo * g.n + v] = amask[vi];
-        push_vertex(c, s, g, lo, v);
-      }
+    for (int b0 = 0; b0 < tv; b0 += BLK) {  // wave-uniform trip count for the aggregated push
+      const int vi = b0 + (int)threadIdx.x;
+      const unsigned long long mk = vi < tv ? amask[vi] : 0ull;
+      const int32_t v = (int32_t)(v0 + vi);
+      if (mk) c.act[(size_t)lo * g.n + v] = mk;
+      push_vertex_wave(c, s, g, lo, mk != 0, v);
     }
     __syncthreads();
   }
@@ -1142,11 +1205,16 @@ __global__ __launch_bounds__(BLK) void k_merge(const JobConst *jcs, JobState *js
 }
 
 // Hop merge for level L: T(v,p,max(d1,d2)) = min T(v,p1,d1) + T(v,p2,d2), d1 + d2 <= D.
-constexpr int HMERGE_LDS_WORDS = 8192;
+// Next to the staged rows the tile keeps their prefix minima over height,
+// pm(d,q) = min_{d' <= d} T(v,q,d'), so a split costs two sums per target height nd:
+// d1 = nd with d2 <= min(nd, D - nd), and d2 = nd with d1 <= min(nd - 1, D - nd).
+constexpr int HMERGE_LDS_WORDS = 4096;
+constexpr int HMERGE_MASKS = HMERGE_LDS_WORDS / 4;   // tv * H <= 4096 / W, W >= 4
 __global__ __launch_bounds__(BLK) void k_merge_hop(const JobConst *jcs, JobState *jss, const Plan *pl, GraphDev g) {
   __shared__ long long soff[MAXJOBS + 1];
   __shared__ int32_t rows[HMERGE_LDS_WORDS];
-  __shared__ unsigned long long amask[2048];
+  __shared__ int32_t pm[HMERGE_LDS_WORDS];
+  __shared__ unsigned long long amask[HMERGE_MASKS];
   const int nj = load_plan(pl, WK_MERGE, soff);
   const long long total = soff[nj];
   for (long long gt = blockIdx.x; gt < total; gt += gridDim.x) {
@@ -1162,57 +1230,65 @@ __global__ __launch_bounds__(BLK) void k_merge_hop(const JobConst *jcs, JobState
   {
     const long long v0 = (gt - soff[jb]) * TV;
     const int tv = (int)min((long long)TV, (long long)g.n - v0);
-    for (int i = threadIdx.x; i < tv * RW; i += BLK) rows[i] = INF;
     for (int i = threadIdx.x; i < tv * H; i += BLK) amask[i] = 0;
-    __syncthreads();
+    // every subset read below has fewer than L members, so staging overwrites all of it
     for (int l = 1; l < L; l++) {
-      const int ns = c.nsub[l];
-      const int32_t *blk = c.T + c.off[l] + (size_t)v0 * H * ns;
-      for (int i = threadIdx.x; i < tv * H * ns; i += BLK) {
-        const int vi = i / (H * ns), rem = i - vi * H * ns;
+      const int ns = c.nsub[l], hn = H * ns;
+      const uint8_t *sub = c.sub[l];
+      stage_block(c.T + c.off[l] + (size_t)v0 * hn, tv * hn, [&](int i, int32_t x) {
+        const int vi = i / hn, rem = i - vi * hn;
         const int d = rem / ns, r = rem - d * ns;
-        rows[vi * RW + d * W + c.sub[l][r]] = blk[i];
+        rows[vi * RW + d * W + sub[r]] = x;
+      });
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < tv * W; i += BLK) {
+      const int vi = i / W, q = i - vi * W;
+      if (q == 0 || popc(q) >= L) continue;
+      int m = INF;
+      for (int d = 0; d < H; d++) {
+        const int x = rows[vi * RW + d * W + q];
+        m = x < m ? x : m;
+        pm[vi * RW + d * W + q] = m;
       }
     }
     __syncthreads();
+    // item order (vertex, height, subset) makes the level-block write below contiguous
     for (int it = threadIdx.x; it < tv * nL * H; it += BLK) {
       const int vi = it / (nL * H), rem = it - vi * nL * H;
-      const int j = rem / H, nd = rem - j * H;
+      const int nd = rem / nL, j = rem - nd * nL;
       const int p = c.sub[L][j];
-      const int32_t *row = rows + vi * RW;
       const int low = p & -p, rest = p ^ low;
-      const int lim = min(nd, D - nd);
+      const int l2 = min(nd, D - nd), l1 = min(nd - 1, D - nd);
+      const int32_t *rn = rows + vi * RW + nd * W;
+      const int32_t *q2 = pm + vi * RW + l2 * W;
+      const int32_t *q1 = pm + vi * RW + max(l1, 0) * W;
       int m = INF;
-      if (lim >= 0) {
-        for (int sm = rest;; sm = (sm - 1) & rest) {
-          const int p1 = low | sm, p2 = p ^ p1;
-          if (p1 != p) {
-            const int a1 = row[nd * W + p1], a2 = row[nd * W + p2];
-            for (int d2 = 0; d2 <= lim; d2++) {  // d1 = nd
-              const int t = a1 + row[d2 * W + p2];
-              m = t < m ? t : m;
-            }
-            for (int d1 = 0; d1 <= lim && d1 < nd; d1++) {  // d2 = nd
-              const int t = row[d1 * W + p1] + a2;
-              m = t < m ? t : m;
-            }
+      for (int sm = rest;; sm = (sm - 1) & rest) {
+        const int p1 = low | sm, p2 = p ^ p1;
+        if (p1 != p) {
+          const int t = rn[p1] + q2[p2];
+          m = t < m ? t : m;
+          if (l1 >= 0) {
+            const int u = q1[p1] + rn[p2];
+            m = u < m ? u : m;
           }
-          if (sm == 0) break;
         }
+        if (sm == 0) break;
       }
       if (m >= INF) continue;
       merged++;
-      c.T[(size_t)c.off[L] + ((size_t)(v0 + vi) * H + nd) * nL + j] = m;
+      c.T[(size_t)c.off[L] + (size_t)v0 * H * nL + it] = m;
       if (L < K && nd < D) atomicOr(&amask[vi * H + nd], 1ull << j);
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < tv * H; i += BLK) {
-      if (amask[i]) {
-        const int vi = i / H, nd = i - vi * H;
-        const int32_t v = (int32_t)(v0 + vi);
-        c.act[(size_t)nd * g.n + v] = amask[i];
-        push_vertex(c, s, g, nd, v);
-      }
+    for (int b0 = 0; b0 < tv * H; b0 += BLK) {  // wave-uniform trip count for the aggregated push
+      const int i = b0 + (int)threadIdx.x;
+      const unsigned long long mk = i < tv * H ? amask[i] : 0ull;
+      const int vi = i / H, nd = i - vi * H;
+      const int32_t v = (int32_t)(v0 + vi);
+      if (mk) c.act[(size_t)nd * g.n + v] = mk;
+      push_vertex_wave(c, s, g, nd, mk != 0, v);
     }
     __syncthreads();
   }
