@@ -56,6 +56,7 @@ struct Slot {
   int32_t *T = nullptr;
   unsigned long long *act = nullptr;
   int32_t *lst = nullptr, *blst = nullptr, *tab = nullptr, *stk = nullptr, *out = nullptr;
+  unsigned long long *stx = nullptr;  // striped work counters
   size_t bytes = 0;
   int nact = 0, nlst = 0, nblst = 0;  // mask arrays, thread lists, wavefront lists
 };
@@ -121,6 +122,7 @@ size_t slot_bytes(const gst_ctx *c, int K, int D) {
   b += (size_t)nblst * c->bcap * 8;                     // wavefront lists
   b += (size_t)(MAXK * MAXK * MAXW + MAXK * MAXW + MAXK * MAXK) * 4;  // tables
   b += (size_t)SCAP * 12 + (size_t)ECAP * 12;
+  b += (size_t)NSTRIPE * NSTX * 8;
   return b;
 }
 
@@ -154,6 +156,7 @@ int slot_alloc(gst_ctx *c, Slot &s, int K, int D) {
   s.tab = (int32_t *)take((size_t)(MAXK * MAXK * MAXW + MAXK * MAXW + MAXK * MAXK) * 4);
   s.stk = (int32_t *)take((size_t)SCAP * 12);
   s.out = (int32_t *)take((size_t)ECAP * 12);
+  s.stx = (unsigned long long *)take((size_t)NSTRIPE * NSTX * 8);
   if (off > s.bytes) {  // alignment slack
     (void)hipFree(base);
     s.bytes = off + 4096;
@@ -167,6 +170,7 @@ int slot_alloc(gst_ctx *c, Slot &s, int K, int D) {
     s.tab = (int32_t *)take((size_t)(MAXK * MAXK * MAXW + MAXK * MAXW + MAXK * MAXK) * 4);
     s.stk = (int32_t *)take((size_t)SCAP * 12);
     s.out = (int32_t *)take((size_t)ECAP * 12);
+    s.stx = (unsigned long long *)take((size_t)NSTRIPE * NSTX * 8);
   }
   return GST_OK;
 }
@@ -194,7 +198,7 @@ int check_query(const gst_ctx *c, const int32_t *gids, int32_t k, int32_t D, int
 
 void fill_jobconst(const gst_ctx *c, const Slot &s, const Query &q, JobConst &jc) {
   std::memset(&jc, 0, sizeof(jc));
-  jc.T = s.T; jc.act = s.act; jc.lst = s.lst; jc.blst = s.blst; jc.tab = s.tab;
+  jc.T = s.T; jc.stx = s.stx; jc.act = s.act; jc.lst = s.lst; jc.blst = s.blst; jc.tab = s.tab;
   jc.stk = s.stk; jc.out = s.out;
   jc.bcap = c->bcap; jc.scap = SCAP; jc.ecap = ECAP;
   jc.K = q.K; jc.D = q.D; jc.H = q.D >= 0 ? q.D + 1 : 1; jc.W = 1 << q.K; jc.R = jc.W - 1;
@@ -243,6 +247,19 @@ inline unsigned grid_for(long long items, long long per_block, unsigned cap) {
 void launch_job_init(hipStream_t st, const Slot &s, const gst_ctx *c) {
   k_fill_i32<<<grid_for((long long)s.arena_words / 4, BLK, 8192), BLK, 0, st>>>(s.T, (long long)s.arena_words, INF);
   k_fill_u64<<<grid_for((long long)s.nact * c->n, BLK, 4096), BLK, 0, st>>>(s.act, (long long)s.nact * c->n, 0ull);
+  k_fill_u64<<<grid_for(NSTRIPE * NSTX, BLK, 64), BLK, 0, st>>>(s.stx, (long long)NSTRIPE * NSTX, 0ull);
+}
+
+// Sum the counter stripes of a finished job into its state record.
+void fold_stats(const unsigned long long *stripes, JobState &js) {
+  unsigned long long t[NSTX] = {0};
+  for (int r = 0; r < NSTRIPE; r++)
+    for (int i = 0; i < NSTX; i++) t[i] += stripes[r * NSTX + i];
+  for (int i = 0; i < 8; i++) js.st[i] = t[i];
+  for (int L = 0; L < 8; L++) {
+    js.stl[L][0] = t[SX_LEVEL + 2 * L];
+    js.stl[L][1] = t[SX_LEVEL + 2 * L + 1];
+  }
 }
 
 void finish_result(const gst_ctx *c, const Query &q, const JobState &js, const std::vector<int32_t> &edges,
@@ -527,7 +544,7 @@ int gst_solve(gst_ctx *c, const int32_t *gids, int32_t k, int32_t D, int32_t fla
         k_final<<<grid_for(((long long)c->n * jc.H + FINAL_CHUNK - 1) / FINAL_CHUNK, 1, 4096), BLK, 0, c->st>>>(c->d_jc, c->d_js, pl, g);
         break;
       case PH_REBUILD:
-        k_rebuild<<<1, BLK, 0, c->st>>>(c->d_jc, c->d_js, 1, g);
+        k_rebuild<<<1, RB_BLK, 0, c->st>>>(c->d_jc, c->d_js, 1, g);
         break;
       default:
         break;
@@ -542,6 +559,9 @@ int gst_solve(gst_ctx *c, const int32_t *gids, int32_t k, int32_t D, int32_t fla
     HIPCHK(hipMemcpy(edges.data(), s.out, sizeof(int32_t) * edges.size(), hipMemcpyDeviceToHost));
   }
   JobState fin = h;
+  std::vector<unsigned long long> stripes((size_t)NSTRIPE * NSTX);
+  HIPCHK(hipMemcpy(stripes.data(), s.stx, stripes.size() * 8, hipMemcpyDeviceToHost));
+  fold_stats(stripes.data(), fin);
   finish_result(c, q, fin, edges, out);
   if (q.flags & GST_F_STATS) {
     HIPCHK(hipMemsetAsync(c->d_count, 0, sizeof(unsigned long long), c->st));

@@ -68,9 +68,15 @@ enum Stat : int32_t {
   ST_GROWR = 6,   // grow rounds
   ST_MERGE = 7    // merged states written
 };
+// Work counters are striped: a block adds to stripe blockIdx % NSTRIPE, so the per-wave
+// flushes of one round spread over NSTRIPE cache lines instead of queueing on one word.
+constexpr int NSTRIPE = 64;
+constexpr int NSTX = 32;       // counters per stripe: Stat values, then per-level pairs
+constexpr int SX_LEVEL = 8;    // level L: edges at SX_LEVEL + 2L, relaxations at SX_LEVEL + 2L + 1
 
 struct JobConst {
   int32_t *T;                 // arena
+  unsigned long long *stx;    // [NSTRIPE][NSTX] work counters, summed by the host
   unsigned long long *act;    // [nl][n] active-subset masks (bit j = j-th subset of the level)
   int32_t *lst;               // [nl][n] thread-per-vertex worklists
   int32_t *blst;              // [nl][bcap][2] (vertex, chunk) wavefront work items
@@ -217,10 +223,11 @@ __device__ __forceinline__ int ub_candidate(const int32_t *sing, int p, int cst,
                                             const JobConst &c, int compv) {
   long long star = cst;
   bool ok = true;
-  for (int i = 0; i < c.K; i++)
-    if (!((p >> i) & 1)) {
+#pragma unroll
+  for (int i = 0; i < MAXK; i++)
+    if (i < c.K && !((p >> i) & 1)) {
       const int x = sing[i];
-      if (x >= INF) { ok = false; break; }
+      ok = ok && x < INF;
       star += x;
     }
   long long best = ok ? star : (long long)INF;
@@ -419,6 +426,11 @@ __device__ __forceinline__ unsigned long long grow_select(const JobConst &c, Job
   const int32_t *hk = c.tab;
   const int32_t *w2 = c.tab + c.K * c.K * c.W;
   const bool prune = !(c.flags & JF_NO_PRUNE);
+  // the group distances and r1 are read once per vertex, not once per state
+  int sl[MAXK];
+#pragma unroll
+  for (int i = 0; i < MAXK; i++) sl[i] = i < c.K ? sing[i] : INF;
+  const int32_t r1 = s->r1;
   unsigned long long gm = 0;
   int ubl = ub;
   while (a) {
@@ -430,15 +442,14 @@ __device__ __forceinline__ unsigned long long grow_select(const JobConst &c, Job
     if (cst >= INF) continue;
     const int q = c.R ^ p;
     const int compv = (q && popc(q) < L) ? c.T[sidx(c, v, q, 0)] : INF;
-    const int cand = ub_candidate(sing, p, cst, L, c, compv);
+    const int cand = ub_candidate(sl, p, cst, L, c, compv);
     if (cand < ubl) {
-      const int cur = load_ub(s);
-      if (cand < cur) atomicMin(&s->ub, cand);
-      ubl = cand < cur ? cand : cur;
+      atomicMin(&s->ub, cand);
+      ubl = cand;
     }
     if (prune) {
       if (cst > (ubl >> 1)) continue;
-      if (L > 1 && cst + lb_tail(sing, s->r1, p, c, hk, w2) > ubl) continue;
+      if (L > 1 && cst + lb_tail(sl, r1, p, c, hk, w2) > ubl) continue;
     }
     if (cst > theta) {  // near-far: wait in the far pile
       defer |= 1ull << j;
@@ -494,27 +505,33 @@ struct GrowCounters {
   __device__ void zero() { verts = edges = states = relax = push = atom = 0; dmin = INF; }
 };
 
+// This block's stripe of counter i of a job.
+__device__ __forceinline__ unsigned long long *stat_at(const JobConst &c, int i) {
+  return c.stx + (blockIdx.x & (NSTRIPE - 1)) * NSTX + i;
+}
+
 // Add a thread's counters to job cj's stats: one atomic per counter per wavefront when
 // all lanes worked for the same job (the common case), per lane otherwise.
-__device__ __forceinline__ void flush_lane(JobState *jss, int cj, const GrowCounters &k) {
+__device__ __forceinline__ void flush_lane(const JobConst *jcs, JobState *jss, int cj, const GrowCounters &k) {
   if (cj < 0) return;
+  const JobConst &c = jcs[cj];
   JobState *s = &jss[cj];
   const int L = s->level & 7;
-  if (k.edges) atomicAdd(&s->stl[L][0], (unsigned long long)k.edges);
-  if (k.relax) atomicAdd(&s->stl[L][1], (unsigned long long)k.relax);
-  if (k.verts) atomicAdd(&s->st[ST_VERTS], (unsigned long long)k.verts);
-  if (k.edges) atomicAdd(&s->st[ST_EDGES], (unsigned long long)k.edges);
-  if (k.states) atomicAdd(&s->st[ST_STATES], (unsigned long long)k.states);
-  if (k.relax) atomicAdd(&s->st[ST_RELAX], (unsigned long long)k.relax);
-  if (k.push) atomicAdd(&s->st[ST_PUSH], (unsigned long long)k.push);
-  if (k.atom) atomicAdd(&s->st[ST_ATOM], (unsigned long long)k.atom);
+  if (k.edges) atomicAdd(stat_at(c, SX_LEVEL + 2 * L), (unsigned long long)k.edges);
+  if (k.relax) atomicAdd(stat_at(c, SX_LEVEL + 2 * L + 1), (unsigned long long)k.relax);
+  if (k.verts) atomicAdd(stat_at(c, ST_VERTS), (unsigned long long)k.verts);
+  if (k.edges) atomicAdd(stat_at(c, ST_EDGES), (unsigned long long)k.edges);
+  if (k.states) atomicAdd(stat_at(c, ST_STATES), (unsigned long long)k.states);
+  if (k.relax) atomicAdd(stat_at(c, ST_RELAX), (unsigned long long)k.relax);
+  if (k.push) atomicAdd(stat_at(c, ST_PUSH), (unsigned long long)k.push);
+  if (k.atom) atomicAdd(stat_at(c, ST_ATOM), (unsigned long long)k.atom);
   if (k.dmin < INF) atomicMin(&s->far_min, k.dmin);
 }
 // Called by every lane of the wavefront (converged).
-__device__ void flush_counters(JobState *jss, int cj, GrowCounters &k) {
+__device__ void flush_counters(const JobConst *jcs, JobState *jss, int cj, GrowCounters &k) {
   const int first = __shfl(cj, 0, 64);
   const bool same = __all(cj == first || cj < 0);
-  if (!same || first < 0) { flush_lane(jss, cj, k); return; }
+  if (!same || first < 0) { flush_lane(jcs, jss, cj, k); return; }
   int dm = k.dmin;
   for (int off = 32; off > 0; off >>= 1) {
     const int o = __shfl_xor(dm, off, 64);
@@ -530,14 +547,15 @@ __device__ void flush_counters(JobState *jss, int cj, GrowCounters &k) {
     v[q] = x;
   }
   if ((threadIdx.x & 63) == 0) {
+    const JobConst &c = jcs[first];
     JobState *s = &jss[first];
     const int L = s->level & 7;
-    if (v[1]) atomicAdd(&s->stl[L][0], v[1]);
-    if (v[3]) atomicAdd(&s->stl[L][1], v[3]);
+    if (v[1]) atomicAdd(stat_at(c, SX_LEVEL + 2 * L), v[1]);
+    if (v[3]) atomicAdd(stat_at(c, SX_LEVEL + 2 * L + 1), v[3]);
     const int map[6] = {ST_VERTS, ST_EDGES, ST_STATES, ST_RELAX, ST_PUSH, ST_ATOM};
 #pragma unroll
     for (int q = 0; q < 6; q++)
-      if (v[q]) atomicAdd(&s->st[map[q]], v[q]);
+      if (v[q]) atomicAdd(stat_at(c, map[q]), v[q]);
     if (dm < INF) atomicMin(&s->far_min, dm);
   }
 }
@@ -738,7 +756,7 @@ __global__ __launch_bounds__(BLK) void k_select(const JobConst *jcs, JobState *j
       }
       if (jb >= 0 && jcs[jb].hop) jb = -1;
       if (jb >= 0) {
-        if (jb != cj) { flush_lane(jss, cj, k); k.zero(); cj = jb; }
+        if (jb != cj) { flush_lane(jcs, jss, cj, k); k.zero(); cj = jb; }
         const JobConst &c = jcs[jb];
         JobState *s = &jss[jb];
         unsigned long long *act = c.act + (size_t)cur_list(c, s) * g.n;
@@ -754,7 +772,7 @@ __global__ __launch_bounds__(BLK) void k_select(const JobConst *jcs, JobState *j
     wp_push(pb, jcs, jss, g, jb, fl, want, v, 0);
   }
   wp_flush(pb, jcs, jss, g);
-  flush_counters(jss, cj, k);
+  flush_counters(jcs, jss, cj, k);
 }
 
 // Near-far: the near frontier is empty and the threshold was raised; move the far
@@ -778,7 +796,7 @@ __global__ __launch_bounds__(BLK) void k_farsplit(const JobConst *jcs, JobState 
     bool wantN = false, wantF = false;
     if (it < total) {
       jb = find_job(soff, nj, it);
-      if (jb != cj) { flush_lane(jss, cj, k); k.zero(); cj = jb; }
+      if (jb != cj) { flush_lane(jcs, jss, cj, k); k.zero(); cj = jb; }
       const JobConst &c = jcs[jb];
       JobState *s = &jss[jb];
       unsigned long long *far = c.act + (size_t)FAR_ACT * g.n;
@@ -799,7 +817,7 @@ __global__ __launch_bounds__(BLK) void k_farsplit(const JobConst *jcs, JobState 
     wp_push(pb, jcs, jss, g, jb, fl, wantF, v, 0);
   }
   wp_flush(pb, jcs, jss, g);
-  flush_counters(jss, cj, k);
+  flush_counters(jcs, jss, cj, k);
 }
 
 // --------------------------------------------------------------------------- grow
@@ -874,7 +892,7 @@ __global__ __launch_bounds__(BLK) void k_grow_thread(const JobConst *jcs, JobSta
     unsigned long long gm = 0;
     if (has) {
       jb = find_job(soff, nj, it);
-      if (jb != cj) { flush_lane(jss, cj, k); k.zero(); cj = jb; }
+      if (jb != cj) { flush_lane(jcs, jss, cj, k); k.zero(); cj = jb; }
       const JobConst &c = jcs[jb];
       JobState *s = &jss[jb];
       const int li = cur_list(c, s);
@@ -929,7 +947,7 @@ __global__ __launch_bounds__(BLK) void k_grow_thread(const JobConst *jcs, JobSta
     }
   }
   wp_flush(pb, jcs, jss, g);
-  flush_counters(jss, cj, k);
+  flush_counters(jcs, jss, cj, k);
 }
 
 // Wavefront per (vertex, edge chunk) for vertices with degree >= split. Each lane
@@ -952,7 +970,7 @@ __global__ __launch_bounds__(BLK) void k_grow_wave(const JobConst *jcs, JobState
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: scalar registers
   for (long long it = (long long)blockIdx.x * (BLK / WAVE) + wave; it < total; it += waves_per_grid) {
     const int jb = find_job(soff, nj, it);
-    if (jb != cj) { flush_lane(jss, cj, k); k.zero(); cj = jb; }
+    if (jb != cj) { flush_lane(jcs, jss, cj, k); k.zero(); cj = jb; }
     const JobConst &c = jcs[jb];
     JobState *s = &jss[jb];
     const int li = cur_list(c, s), lo = nxt_list(c, s);
@@ -1001,7 +1019,7 @@ __global__ __launch_bounds__(BLK) void k_grow_wave(const JobConst *jcs, JobState
     }
   }
   wp_flush(pb, jcs, jss, g);
-  flush_counters(jss, cj, k);
+  flush_counters(jcs, jss, cj, k);
 }
 
 // Zero the active masks of the list just consumed (after both grow kernels ran), or of
@@ -1200,7 +1218,7 @@ __global__ __launch_bounds__(BLK) void k_merge(const JobConst *jcs, JobState *js
     }
     __syncthreads();
   }
-  if (merged) atomicAdd(&s->st[ST_MERGE], (unsigned long long)merged);
+  if (merged) atomicAdd(stat_at(c, ST_MERGE), (unsigned long long)merged);
   }
 }
 
@@ -1292,7 +1310,7 @@ __global__ __launch_bounds__(BLK) void k_merge_hop(const JobConst *jcs, JobState
     }
     __syncthreads();
   }
-  if (merged) atomicAdd(&s->st[ST_MERGE], (unsigned long long)merged);
+  if (merged) atomicAdd(stat_at(c, ST_MERGE), (unsigned long long)merged);
   }
 }
 
@@ -1347,7 +1365,9 @@ __device__ __forceinline__ bool in_group(const GraphDev &g, int grp, int32_t v) 
   return lo < g.goffs[grp + 1] && g.gverts[lo] == v;
 }
 
-__global__ __launch_bounds__(BLK) void k_rebuild(const JobConst *jcs, JobState *jss, int njobs, GraphDev g) {
+constexpr int RB_BLK = 1024;  // rebuild block: one per job, wide for hub adjacency scans
+constexpr int RB_U = 8;       // arcs per thread per scan chunk
+__global__ __launch_bounds__(RB_BLK) void k_rebuild(const JobConst *jcs, JobState *jss, int njobs, GraphDev g) {
   if ((int)blockIdx.x >= njobs) return;
   const JobConst &c = jcs[blockIdx.x];
   JobState *s = &jss[blockIdx.x];
@@ -1385,7 +1405,7 @@ __global__ __launch_bounds__(BLK) void k_rebuild(const JobConst *jcs, JobState *
     // merge search: split index = (submask sm of rest) * H*H + d1*H + d2
     const int low = p & -p, rest = p ^ low;
     const int nsm = 1 << popc(rest);
-    for (int it = threadIdx.x; it < nsm * H * H; it += BLK) {
+    for (int it = threadIdx.x; it < nsm * H * H; it += RB_BLK) {
       const int k = it / (H * H), dd = it - k * H * H, d1 = dd / H, d2 = dd - d1 * H;
       // k-th submask of rest (pdep of k into rest)
       int sm = 0, kk = k;
@@ -1425,17 +1445,24 @@ __global__ __launch_bounds__(BLK) void k_rebuild(const JobConst *jcs, JobState *
     }
     const int pd = c.hop ? d - 1 : 0;
     const int32_t e0 = g.beg[v], e1 = g.beg[v + 1];
-    // ascending chunks of 4*BLK edges; the first chunk holding a predecessor has the
-    // smallest index, so the scan stops there (hubs have millions of edges)
-    for (int32_t cb = e0; cb < e1; cb += 4 * BLK) {
+    // ascending chunks of RB_U*RB_BLK arcs; the first chunk holding a predecessor has
+    // the smallest index, so the scan stops there (hubs have millions of arcs). All
+    // loads of a chunk are issued before the first comparison.
+    for (int32_t cb = e0; cb < e1; cb += RB_U * RB_BLK) {
+      int32_t uu[RB_U], ww[RB_U], tt[RB_U];
 #pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const int32_t e = cb + r * BLK + threadIdx.x;
-        if (e < e1) {
-          const int32_t u = g.adj[e];
-          const long long t = (long long)c.T[sidx(c, u, p, pd)] + g.w[e];
-          if (t <= cost) atomicMin((unsigned long long *)&found, (unsigned long long)(e - e0));
-        }
+      for (int r = 0; r < RB_U; r++) {
+        const int32_t e = cb + r * RB_BLK + (int32_t)threadIdx.x;
+        uu[r] = e < e1 ? g.adj[e] : -1;
+        ww[r] = e < e1 ? g.w[e] : 0;
+      }
+#pragma unroll
+      for (int r = 0; r < RB_U; r++) tt[r] = uu[r] >= 0 ? c.T[sidx(c, uu[r], p, pd)] : INF;
+#pragma unroll
+      for (int r = 0; r < RB_U; r++) {
+        const int32_t e = cb + r * RB_BLK + (int32_t)threadIdx.x;
+        if (uu[r] >= 0 && (long long)tt[r] + ww[r] <= cost)
+          atomicMin((unsigned long long *)&found, (unsigned long long)(e - e0));
       }
       __syncthreads();
       if (found != 0x7fffffffffffffffll) break;
