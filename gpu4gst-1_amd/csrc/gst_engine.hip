@@ -236,6 +236,12 @@ void init_jobstate(JobState &js) {
   js.minc = INF;
 }
 
+// Grid of the select and grow launches: one wave of resident blocks. Fewer waves means
+// fewer end-of-launch flushes onto the shared worklist counters.
+#ifndef GROW_GRID_CAP
+#define GROW_GRID_CAP 1024
+#endif
+
 inline unsigned grid_for(long long items, long long per_block, unsigned cap) {
   long long b = (items + per_block - 1) / per_block;
   if (b < 1) b = 1;
@@ -512,9 +518,9 @@ int gst_solve(gst_ctx *c, const int32_t *gids, int32_t k, int32_t D, int32_t fla
         const int li = q.D >= 0 ? h.depth : h.parity;
         const int n0 = h.cnt[li][0], n1 = h.cnt[li][1];
         HIPCHK(hipEventRecord(c->ev0, c->st));
-        if (q.D < 0) k_select<<<grid_for(n0 + n1, BLK, 1 << 16), BLK, 0, c->st>>>(c->d_jc, c->d_js, pl, g);
-        if (n0) k_grow_thread<<<grid_for(n0, BLK, 1 << 20), BLK, 0, c->st>>>(c->d_jc, c->d_js, pl, g);
-        if (n1) k_grow_wave<<<grid_for(n1, BLK / WAVE, 1 << 20), BLK, 0, c->st>>>(c->d_jc, c->d_js, pl, g);
+        if (q.D < 0) k_select<<<grid_for(n0 + n1, BLK, GROW_GRID_CAP), BLK, 0, c->st>>>(c->d_jc, c->d_js, pl, g);
+        if (n0) k_grow_thread<<<grid_for(n0, BLK, GROW_GRID_CAP), BLK, 0, c->st>>>(c->d_jc, c->d_js, pl, g);
+        if (n1) k_grow_wave<<<grid_for(n1, BLK / WAVE, GROW_GRID_CAP), BLK, 0, c->st>>>(c->d_jc, c->d_js, pl, g);
         HIPCHK(hipEventRecord(c->ev1, c->st));
         timed_phase = PH_GROW;
         k_clear<<<grid_for(n0 + n1, BLK, 1 << 16), BLK, 0, c->st>>>(c->d_jc, c->d_js, pl, g);
