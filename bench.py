@@ -7,8 +7,8 @@ undirected edges, largest connected component, int32 weights uniform in [1,100],
 A step = one batch of --batch queries solved by the batched persistent path
 (gst_solve_batch). value = queries/s over the timed steps (inputs resident in HBM).
 Beside it: the plain single-query path on --plain-queries of the same queries (q/s,
-p50/p99), the grow-kernel roofline (algorithmic bytes / HIP-event time of the grow
-kernels, plain path), and the CPU PrunedDP++ restatement timed on the host cores on a
+p50/p99), the grow-round roofline (algorithmic bytes / HIP-event time of the select and
+grow kernels of each round, plain path), and the CPU PrunedDP++ restatement timed on the host cores on a
 bounded sample, whose costs also serve as the parity check of the GPU results.
 """
 import argparse
@@ -99,6 +99,22 @@ def serve(args):
     eng.close()
 
 
+def calibrate_copy(mib):
+    """One device-to-device copy of a known size: under rocprofv3 --pmc FETCH_SIZE (or
+    WRITE_SIZE) its dispatch should read (write) mib * 1024 KiB, which calibrates the
+    counters used for the grow kernels' HBM traffic."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    src = torch.arange(n, dtype=torch.int32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    assert bool((dst[-4:] == src[-4:]).all())
+    print(json.dumps({"calibrate_copy_mib": mib, "expected_kib_each_way": mib * 1024}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -112,12 +128,18 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-only", action="store_true", help="plain path only (for rocprof runs)")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (library defaults)")
+    ap.add_argument("--tuning", type=int, nargs=2, metavar=("SPLIT", "CHUNK"),
+                    help="skip the autotuner and use this degree split and chunk size")
     ap.add_argument("--stats", action="store_true", help="count finite states (extra arena scan per query)")
     ap.add_argument("--serve", action="store_true", help="served form with Poisson arrivals instead")
     ap.add_argument("--rates", type=float, nargs="+", default=[2.0, 8.0, 32.0])
     ap.add_argument("--serve-queries", type=int, default=32)
     ap.add_argument("--budget-gb", type=float, default=200.0)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device to device once (HBM counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        return calibrate_copy(args.calibrate_copy)
     if args.serve and args.gpus == 1:
         if args.workload == "cl3m":
             args.workload = "cl4m"
@@ -145,7 +167,10 @@ def main():
     eng = engine.Engine(g, wl.groups, max_batch=args.batch)
     print(f"# opened engine ({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
     tuned = None
-    if not args.no_tune:
+    if args.tuning:
+        eng.set_tuning(args.tuning[0], args.tuning[1], 0)
+        tuned = {"degree_split": args.tuning[0], "chunk_edges": args.tuning[1], "near_far_delta": 0}
+    elif not args.no_tune:
         sample = queries[-3:]
         tuned, _ = autotune.tune(eng, sample, grid={"degree_split": (128, 256, 512), "chunk_edges": (4096, 16384),
                                                    "near_far_delta": (0,)}, budget_s=30.0)
@@ -164,6 +189,7 @@ def main():
     gb = sum(b_grow(r) for r in plain)
     gt = sum(r["t_grow_s"] for r in plain)
     grow_gbs = gb / gt / 1e9 if gt > 0 else 0.0
+    grow_rounds = max(sum(r["grow_rounds"] for r in plain), 1)
     print(f"# plain: {plain_qps:.2f} q/s p50 {np.median(lat):.2f} ms p99 {np.percentile(lat, 99):.2f} ms; "
           f"grow {grow_gbs:.1f} GB/s ({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
     if args.profile_only:
@@ -247,7 +273,10 @@ def main():
         "parity_batched_vs_plain": f"{plain_match}/{len(plain)}",
         "roofline": {"bound": "hbm", "achieved": round(grow_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(grow_gbs / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_grow_thread + k_grow_wave (plain path, HIP events)"},
+                     "kernel": "grow round: k_select + k_grow_thread + k_grow_wave (plain path, HIP events "
+                               "on the engine stream)",
+                     "launches": grow_rounds, "avg_launch_us": round(gt / grow_rounds * 1e6, 1),
+                     "bytes_per_launch": int(gb / grow_rounds)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

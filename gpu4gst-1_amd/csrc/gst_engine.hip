@@ -274,6 +274,7 @@ void finish_result(const gst_ctx *c, const Query &q, const JobState &js, const s
   out->k = q.K;
   out->D = q.D;
   out->rounds = js.rounds;
+  out->grow_rounds = js.dp_rounds;
   out->states_processed = (int64_t)js.st[ST_STATES];
   out->grow_vertices = (int64_t)js.st[ST_VERTS];
   out->grow_edges = (int64_t)js.st[ST_EDGES];

@@ -54,7 +54,7 @@ class GstResult(ctypes.Structure):
                 ("states_processed", ctypes.c_int64), ("grow_vertices", ctypes.c_int64),
                 ("grow_edges", ctypes.c_int64), ("grow_relax", ctypes.c_int64), ("grow_pushes", ctypes.c_int64),
                 ("arena_bytes", ctypes.c_int64), ("level_edges", ctypes.c_int64 * 8),
-                ("level_relax", ctypes.c_int64 * 8)]
+                ("level_relax", ctypes.c_int64 * 8), ("grow_rounds", ctypes.c_int32)]
 
 
 class GstInfo(ctypes.Structure):

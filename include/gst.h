@@ -117,6 +117,7 @@ typedef struct {
   int64_t arena_bytes;    /* DP arena size V*2^k*(D+1)*4 */
   int64_t level_edges[8]; /* grow work per DP level (subset size 1..k): edges scanned */
   int64_t level_relax[8]; /*   and state-edge relaxations */
+  int32_t grow_rounds;    /* grow rounds executed (one select + grow launch set each) */
 } gst_result;
 
 typedef struct {

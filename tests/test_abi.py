@@ -85,3 +85,26 @@ def test_open_rejects_malformed_input():
 @pytest.mark.skipif(os.path.exists("/dev/kfd") and os.access("/dev/kfd", os.R_OK), reason="a GPU is present")
 def test_open_without_device():
     assert _open([0, 1, 2], [1, 0], [5, 5], [0, 1], [0]) == engine.GST_NO_DEVICE
+
+
+@pytest.mark.parametrize("ctype, cname", [("GstResult", "gst_result"), ("GstOpts", "gst_opts"),
+                                          ("GstCsr", "gst_csr"), ("GstGroups", "gst_groups"),
+                                          ("GstQuery", "gst_query"), ("GstInfo", "gst_info")])
+def test_ctypes_mirror_matches_header_layout(tmp_path, ctype, cname):
+    """Every field of the ctypes mirror sits at the offset the C compiler gives it."""
+    cls = getattr(engine, ctype)
+    fields = [f for f, _ in cls._fields_]
+    src = tmp_path / "layout.c"
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "gst.h"', 'int main(void) {',
+             f'  printf("size %zu\\n", sizeof({cname}));']
+    lines += [f'  printf("{f} %zu\\n", offsetof({cname}, {f}));' for f in fields]
+    lines += ['  return 0;', '}']
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    import subprocess
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(src)], check=True)
+    out = dict(line.split() for line in subprocess.run([str(exe)], check=True, capture_output=True,
+                                                       text=True).stdout.splitlines())
+    assert int(out["size"]) == ctypes.sizeof(cls)
+    for f in fields:
+        assert int(out[f]) == getattr(cls, f).offset, f
