@@ -100,19 +100,24 @@ def serve(args):
 
 
 def calibrate_copy(mib):
-    """One device-to-device copy of a known size: under rocprofv3 --pmc FETCH_SIZE (or
-    WRITE_SIZE) its dispatch should read (write) mib * 1024 KiB, which calibrates the
-    counters used for the grow kernels' HBM traffic."""
+    """Two device-to-device copies of a known size, for the HBM byte counters: the first
+    reads a region that 3 * mib MiB of later writes pushed out of the 256 MB Infinity
+    Cache, the second reads the region written last. Under rocprofv3 --pmc FETCH_SIZE (or
+    WRITE_SIZE) each copy dispatch should read (write) mib * 1024 KiB."""
     import torch
 
     n = mib * (1 << 20) // 4
-    src = torch.arange(n, dtype=torch.int32, device="cuda")
-    dst = torch.empty_like(src)
+    src = torch.empty(4 * n, dtype=torch.int32, device="cuda")
+    src.fill_(7)
+    dst = torch.empty(n, dtype=torch.int32, device="cuda")
     torch.cuda.synchronize()
-    dst.copy_(src)
+    dst.copy_(src[:n])           # cold source
     torch.cuda.synchronize()
-    assert bool((dst[-4:] == src[-4:]).all())
-    print(json.dumps({"calibrate_copy_mib": mib, "expected_kib_each_way": mib * 1024}))
+    dst.copy_(src[3 * n:])       # source written last
+    torch.cuda.synchronize()
+    assert bool((dst[-4:] == 7).all())
+    print(json.dumps({"calibrate_copy_mib": mib, "expected_kib_each_way": mib * 1024,
+                      "copies": ["cold source", "recently written source"]}))
 
 
 def main():

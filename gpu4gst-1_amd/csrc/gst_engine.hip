@@ -83,7 +83,8 @@ struct gst_ctx {
   int32_t *d_nchunk = nullptr;  // per vertex: wavefront chunks (0 = thread item)
   long long *d_goffs = nullptr;
   std::vector<int64_t> h_goffs;
-  std::vector<int32_t> h_deg;
+  std::vector<int32_t> h_deg;       // degrees in device order
+  std::vector<int32_t> old_of_new;  // device vertex -> caller's id (empty: same numbering)
   int32_t n_groups = 0;
   int64_t graph_bytes = 0;
   long long bcap = 0;  // wavefront items one worklist can hold
@@ -192,7 +193,7 @@ int check_query(const gst_ctx *c, const int32_t *gids, int32_t k, int32_t D, int
   if (q.K > MAXK) { set_err("k > GST_MAX_K"); return GST_BAD_INPUT; }
   if (D > GST_MAX_D) { set_err("D > GST_MAX_D"); return GST_BAD_INPUT; }
   q.D = D < 0 ? -1 : D;
-  q.flags = flags | c->opts.flags;
+  q.flags = flags | (c->opts.flags & ~GST_O_KEEP_ORDER);
   return GST_OK;
 }
 
@@ -304,12 +305,15 @@ void finish_result(const gst_ctx *c, const Query &q, const JobState &js, const s
   const long long idx = (long long)(js.ans & 0xffffffffull);
   out->status = GST_OK;
   out->cost = (int32_t)(js.ans >> 32);
-  out->root = (int32_t)(idx / H);
+  auto ext = [&](int32_t v) { return c->old_of_new.empty() ? v : c->old_of_new[v]; };
+  out->root = ext((int32_t)(idx / H));
   out->root_height = q.D >= 0 ? (int32_t)(idx % H) : -1;
-  // distinct edges, sorted
+  // distinct edges in the caller's ids, sorted
   std::vector<std::array<int32_t, 3>> e;
-  for (size_t i = 0; i + 2 < edges.size(); i += 3)
-    e.push_back({edges[i], edges[i + 1], edges[i + 2]});
+  for (size_t i = 0; i + 2 < edges.size(); i += 3) {
+    const int32_t a = ext(edges[i]), b = ext(edges[i + 1]);
+    e.push_back({std::min(a, b), std::max(a, b), edges[i + 2]});
+  }
   std::sort(e.begin(), e.end());
   e.erase(std::unique(e.begin(), e.end(), [](const std::array<int32_t, 3> &a, const std::array<int32_t, 3> &b) {
             return a[0] == b[0] && a[1] == b[1];
@@ -329,6 +333,74 @@ void finish_result(const gst_ctx *c, const Query &q, const JobState &js, const s
 }  // namespace
 
 #include "gst_batch.inc"
+
+namespace {
+
+// The graph as the device holds it. With renumbering, device vertex i is the caller's
+// vertex old_of_new[i], taken in order of descending degree (ties by id), and every
+// adjacency list and group is re-sorted by device id. Otherwise the caller's arrays are
+// used as they are (adj, w, gverts point at them).
+struct DeviceGraph {
+  std::vector<int32_t> beg, old_of_new;
+  std::vector<int32_t> adj_v, w_v, gverts_v;
+  const int32_t *adj = nullptr, *w = nullptr, *gverts = nullptr;
+};
+
+template <class F>
+void parallel_ranges(int64_t n, F f) {
+  const int nt = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  std::vector<std::thread> th;
+  for (int t = 0; t < nt; t++)
+    th.emplace_back([=] { f(n * t / nt, n * (t + 1) / nt); });
+  for (auto &x : th) x.join();
+}
+
+DeviceGraph renumber(const gst_csr *csr, const gst_groups *groups, bool reorder) {
+  const int32_t n = csr->n;
+  const int64_t m = csr->m;
+  DeviceGraph d;
+  d.beg.resize(n + 1);
+  const int64_t total = groups->n_groups ? groups->offs[groups->n_groups] : 0;
+  if (!reorder) {
+    for (int32_t v = 0; v <= n; v++) d.beg[v] = (int32_t)csr->beg_pos[v];
+    d.adj = csr->adj; d.w = csr->w; d.gverts = groups->verts;
+    return d;
+  }
+  d.old_of_new.resize(n);
+  for (int32_t v = 0; v < n; v++) d.old_of_new[v] = v;
+  auto deg = [&](int32_t v) { return csr->beg_pos[v + 1] - csr->beg_pos[v]; };
+  std::stable_sort(d.old_of_new.begin(), d.old_of_new.end(),
+                   [&](int32_t a, int32_t b) { return deg(a) > deg(b); });
+  std::vector<int32_t> new_of_old(n);
+  for (int32_t i = 0; i < n; i++) new_of_old[d.old_of_new[i]] = i;
+  d.beg[0] = 0;
+  for (int32_t i = 0; i < n; i++) d.beg[i + 1] = d.beg[i] + (int32_t)deg(d.old_of_new[i]);
+  d.adj_v.resize(m);
+  d.w_v.resize(m);
+  parallel_ranges(n, [&](int64_t lo, int64_t hi) {
+    std::vector<uint64_t> key;
+    for (int64_t i = lo; i < hi; i++) {
+      const int32_t o = d.old_of_new[i];
+      const int64_t b = csr->beg_pos[o], e = csr->beg_pos[o + 1];
+      key.resize(e - b);
+      for (int64_t t = b; t < e; t++)
+        key[t - b] = ((uint64_t)(uint32_t)new_of_old[csr->adj[t]] << 32) | (uint32_t)csr->w[t];
+      std::sort(key.begin(), key.end());
+      for (int64_t t = 0; t < e - b; t++) {
+        d.adj_v[d.beg[i] + t] = (int32_t)(key[t] >> 32);
+        d.w_v[d.beg[i] + t] = (int32_t)(key[t] & 0xffffffffu);
+      }
+    }
+  });
+  d.gverts_v.resize(total);
+  for (int32_t g = 0; g < groups->n_groups; g++) {
+    for (int64_t t = groups->offs[g]; t < groups->offs[g + 1]; t++) d.gverts_v[t] = new_of_old[groups->verts[t]];
+    std::sort(d.gverts_v.begin() + groups->offs[g], d.gverts_v.begin() + groups->offs[g + 1]);
+  }
+  return d;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -397,10 +469,11 @@ int gst_open(const gst_csr *csr, const gst_groups *groups, const gst_opts *opts,
   c->chunk = c->opts.chunk_edges > 0 ? ((c->opts.chunk_edges + 63) / 64) * 64 : 4096;
   c->delta = c->opts.near_far_delta > 0 ? c->opts.near_far_delta : 0;
   c->max_batch = c->opts.max_batch > 0 ? c->opts.max_batch : 64;
-  std::vector<int32_t> beg32(n + 1);
+  DeviceGraph dg = renumber(csr, groups, !(c->opts.flags & GST_O_KEEP_ORDER));
+  const std::vector<int32_t> &beg32 = dg.beg;
   c->h_deg.resize(n);
-  for (int32_t v = 0; v <= n; v++) beg32[v] = (int32_t)csr->beg_pos[v];
   for (int32_t v = 0; v < n; v++) c->h_deg[v] = beg32[v + 1] - beg32[v];
+  c->old_of_new = std::move(dg.old_of_new);
   c->h_goffs.assign(groups->offs, groups->offs + groups->n_groups + 1);
   if (groups->n_groups == 0) c->h_goffs.assign(1, 0);
   HIPCHK(hipMalloc((void **)&c->d_beg, sizeof(int32_t) * (n + 1)));
@@ -411,14 +484,15 @@ int gst_open(const gst_csr *csr, const gst_groups *groups, const gst_opts *opts,
   HIPCHK(hipMalloc((void **)&c->d_nchunk, sizeof(int32_t) * n));
   HIPCHK(hipMemcpy(c->d_beg, beg32.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice));
   if (m) {
-    HIPCHK(hipMemcpy(c->d_adj, csr->adj, sizeof(int32_t) * m, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->d_w, csr->w, sizeof(int32_t) * m, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_adj, dg.adj ? dg.adj : dg.adj_v.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_w, dg.w ? dg.w : dg.w_v.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice));
   }
   {
     std::vector<long long> go(c->h_goffs.begin(), c->h_goffs.end());
     HIPCHK(hipMemcpy(c->d_goffs, go.data(), sizeof(long long) * go.size(), hipMemcpyHostToDevice));
   }
-  if (total) HIPCHK(hipMemcpy(c->d_gverts, groups->verts, sizeof(int32_t) * total, hipMemcpyHostToDevice));
+  if (total) HIPCHK(hipMemcpy(c->d_gverts, dg.gverts ? dg.gverts : dg.gverts_v.data(), sizeof(int32_t) * total,
+                              hipMemcpyHostToDevice));
   c->graph_bytes = (int64_t)4 * (n + 1) + 8 * m + 8 * (int64_t)c->h_goffs.size() + 4 * total;
   HIPCHK(hipMalloc((void **)&c->d_jc, sizeof(JobConst) * c->max_batch));
   HIPCHK(hipMalloc((void **)&c->d_js, sizeof(JobState) * c->max_batch));

@@ -1129,6 +1129,21 @@ __device__ __forceinline__ void stage_block(const int32_t *__restrict__ src, int
   }
 }
 
+// Merge counters are kept per thread across the tiles of one job and flushed when the
+// block moves to another job or ends: one atomic per wavefront, not one per thread and
+// tile. Called by the whole block (cj is block-uniform).
+__device__ __forceinline__ void flush_merge(const JobConst *jcs, JobState *jss, int cj,
+                                            unsigned long long &merged, int &minc) {
+  if (cj >= 0) {
+    unsigned long long x = merged;
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    if ((threadIdx.x & 63) == 0 && x) atomicAdd(stat_at(jcs[cj], ST_MERGE), x);
+    if (threadIdx.x == 0 && minc < INF) atomicMin(&jss[cj].minc, minc);
+  }
+  merged = 0;
+  minc = INF;
+}
+
 // Non-hop merge for level L: T(v,p) = min over splits p = p1 + p2 of T(v,p1) + T(v,p2).
 // A tile of vertices has its rows staged in LDS (coalesced contiguous read), each
 // thread evaluates (vertex, subset) items, new states are activated for growth.
@@ -1140,8 +1155,11 @@ __global__ __launch_bounds__(BLK) void k_merge(const JobConst *jcs, JobState *js
   __shared__ int ubs, minc_s;
   const int nj = load_plan(pl, WK_MERGE, soff);
   const long long total = soff[nj];
+  int cj = -1, minc_acc = INF;
+  unsigned long long merged_acc = 0;
   for (long long gt = blockIdx.x; gt < total; gt += gridDim.x) {
   const int jb = find_job(soff, nj, gt);
+  if (jb != cj) { flush_merge(jcs, jss, cj, merged_acc, minc_acc); cj = jb; }
   const JobConst &c = jcs[jb];
   JobState *s = &jss[jb];
   if (c.hop) continue;
@@ -1208,7 +1226,7 @@ __global__ __launch_bounds__(BLK) void k_merge(const JobConst *jcs, JobState *js
     if (mact < INF) atomicMin(&minc_s, mact);
     __syncthreads();
     if (threadIdx.x == 0 && ubs < ub) atomicMin(&s->ub, ubs);
-    if (threadIdx.x == 0 && minc_s < INF) atomicMin(&s->minc, minc_s);
+    if (threadIdx.x == 0) minc_acc = minc_s < minc_acc ? minc_s : minc_acc;
     for (int b0 = 0; b0 < tv; b0 += BLK) {  // wave-uniform trip count for the aggregated push
       const int vi = b0 + (int)threadIdx.x;
       const unsigned long long mk = vi < tv ? amask[vi] : 0ull;
@@ -1218,8 +1236,9 @@ __global__ __launch_bounds__(BLK) void k_merge(const JobConst *jcs, JobState *js
     }
     __syncthreads();
   }
-  if (merged) atomicAdd(stat_at(c, ST_MERGE), (unsigned long long)merged);
+  merged_acc += merged;
   }
+  flush_merge(jcs, jss, cj, merged_acc, minc_acc);
 }
 
 // Hop merge for level L: T(v,p,max(d1,d2)) = min T(v,p1,d1) + T(v,p2,d2), d1 + d2 <= D.
@@ -1235,8 +1254,11 @@ __global__ __launch_bounds__(BLK) void k_merge_hop(const JobConst *jcs, JobState
   __shared__ unsigned long long amask[HMERGE_MASKS];
   const int nj = load_plan(pl, WK_MERGE, soff);
   const long long total = soff[nj];
+  int cj = -1, minc_acc = INF;
+  unsigned long long merged_acc = 0;
   for (long long gt = blockIdx.x; gt < total; gt += gridDim.x) {
   const int jb = find_job(soff, nj, gt);
+  if (jb != cj) { flush_merge(jcs, jss, cj, merged_acc, minc_acc); cj = jb; }
   const JobConst &c = jcs[jb];
   JobState *s = &jss[jb];
   if (!c.hop) continue;
@@ -1310,8 +1332,9 @@ __global__ __launch_bounds__(BLK) void k_merge_hop(const JobConst *jcs, JobState
     }
     __syncthreads();
   }
-  if (merged) atomicAdd(stat_at(c, ST_MERGE), (unsigned long long)merged);
+  merged_acc += merged;
   }
+  flush_merge(jcs, jss, cj, merged_acc, minc_acc);
 }
 
 // --------------------------------------------------------------------------- final

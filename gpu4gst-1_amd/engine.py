@@ -22,6 +22,7 @@ _LIB = None
 GST_OK, GST_NO_SOLUTION, GST_PENDING = 0, 1, 2
 GST_BAD_INPUT, GST_OVER_BUDGET, GST_DEVICE_ERROR, GST_INTERNAL, GST_NO_DEVICE = -1, -2, -3, -4, -5
 F_STATS, F_NO_TREE, F_NO_PRUNE = 1, 2, 4
+O_KEEP_ORDER = 0x100  # gst_opts.flags: keep the caller's vertex numbering on the device
 MAX_K, MAX_D = 7, 15
 
 

@@ -60,6 +60,10 @@ enum {
 #define GST_F_STATS 1    /* count finite states after the solve (extra arena scan) */
 #define GST_F_NO_TREE 2  /* skip tree reconstruction */
 #define GST_F_NO_PRUNE 4 /* disable upper/lower-bound pruning (exact either way) */
+/* gst_opts.flags only: keep the caller's vertex numbering on the device. By default
+ * gst_open renumbers vertices by descending degree (hubs first, so the rows that most
+ * arcs point at share a few cache lines); results are always in the caller's ids. */
+#define GST_O_KEEP_ORDER 0x100
 
 typedef struct {
   int32_t n;              /* vertices */

@@ -70,3 +70,28 @@ def test_rmat_hop_k5(gpu_engine, rmat, rmat_inst):
             assert r["cost"] == o["tree_cost"], (q, D, r["cost"], o["tree_cost"])
             ok, why = engine.validate_tree(rmat.graph, rmat.groups, q, r["edges"], r["cost"], D)
             assert ok, (q, D, why)
+
+
+def test_caller_order_matches_renumbered(gpu_engine, rmat, rmat_inst):
+    """gst_open renumbers vertices by degree unless GST_O_KEEP_ORDER is set; both engines
+    return the same optimum, and every tree, root included, is in the caller's ids."""
+    keep = engine.Engine(rmat.graph, rmat.groups, device=0, flags=engine.O_KEEP_ORDER)
+    try:
+        rng = np.random.default_rng(11)
+        for i in range(6):
+            k = int(rng.integers(2, 6))
+            D = int(rng.choice([-1, 2, 4]))
+            q = rng.choice(rmat.groups.n_groups, k, replace=False).astype(np.int32)
+            print("order query", i, k, D, q, flush=True)
+            a = gpu_engine.solve(q, D=D)
+            b = keep.solve(q, D=D)
+            assert a["status"] == b["status"] and a["cost"] == b["cost"], (q, D, a["cost"], b["cost"])
+            if a["status"] != engine.GST_OK:
+                continue
+            for r in (a, b):
+                ok, why = engine.validate_tree(rmat.graph, rmat.groups, q, r["edges"], r["cost"], D)
+                assert ok, (q, D, why)
+                if len(r["edges"]):
+                    assert r["root"] in set(r["edges"][:, :2].ravel().tolist()), (q, D, r["root"])
+    finally:
+        keep.close()
