@@ -609,30 +609,27 @@ __device__ void wp_flush(WavePush &b, const JobConst *jcs, JobState *jss, const 
   if (b.jb >= 0 && (b.ns || b.nb)) {
     const JobConst &c = jcs[b.jb];
     JobState *s = &jss[b.jb];
+    // chunk offsets of the buffered hub vertices, then both ranges reserved by one
+    // atomic instruction (lane 0: thread items, lane 1: wavefront items)
+    const int nch = lane < b.nb ? b.bn[lane] : 0;
+    int x = nch;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int y = __shfl_up(x, off, 64);
+      if (lane >= off) x += y;
+    }
+    const int tot = __shfl(x, 63, 64);
+    int r = 0;
+    if ((lane == 0 && b.ns) || (lane == 1 && b.nb)) r = atomicAdd(&s->cnt[b.lo][lane], lane ? tot : b.ns);
+    const int base_s = __shfl(r, 0, 64), base_b = __shfl(r, 1, 64);
     if (b.ns) {
-      int base = 0;
-      if (lane == 0) base = atomicAdd(&s->cnt[b.lo][0], b.ns);
-      base = __shfl(base, 0, 64);
-      int32_t *dst = c.lst + (size_t)b.lo * g.n + base;
+      int32_t *dst = c.lst + (size_t)b.lo * g.n + base_s;
       for (int i = lane; i < b.ns; i += WAVE) dst[i] = b.sm[i];
     }
-    if (b.nb) {
-      const int nch = lane < b.nb ? b.bn[lane] : 0;
-      int x = nch;
-#pragma unroll
-      for (int off = 1; off < 64; off <<= 1) {
-        const int y = __shfl_up(x, off, 64);
-        if (lane >= off) x += y;
-      }
-      const int tot = __shfl(x, 63, 64);
-      int base = 0;
-      if (lane == 0) base = atomicAdd(&s->cnt[b.lo][1], tot);
-      base = __shfl(base, 0, 64);
-      if (lane < b.nb) {
-        const int32_t v = b.bv[lane];
-        int32_t *dst = c.blst + ((size_t)b.lo * c.bcap + base + x - nch) * 2;
-        for (int i = 0; i < nch; i++) { dst[2 * i] = v; dst[2 * i + 1] = i; }
-      }
+    if (lane < b.nb) {
+      const int32_t v = b.bv[lane];
+      int32_t *dst = c.blst + ((size_t)b.lo * c.bcap + base_b + x - nch) * 2;
+      for (int i = 0; i < nch; i++) { dst[2 * i] = v; dst[2 * i + 1] = i; }
     }
   }
   b.ns = b.nb = 0;
@@ -673,8 +670,10 @@ __device__ __forceinline__ void wp_push(WavePush &b, const JobConst *jcs, JobSta
   b.nb += cb;
 }
 
-// push_vertex for every lane with want, one counter atomic per (list, kind) present in
-// the wavefront instead of one per vertex. li may differ between lanes. Called converged.
+// push_vertex for every lane with want, li may differ between lanes. Lanes are grouped
+// by (list, kind) with ballots and scans only; then every group's lowest lane reserves
+// the group's range with one atomicAdd, all groups in the same instruction, so the wave
+// waits for one memory round trip. Called converged.
 __device__ void push_vertex_wave(const JobConst &c, JobState *s, const GraphDev &g, int li,
                                  bool want, int32_t u) {
   const int lane = lane_id();
@@ -683,36 +682,35 @@ __device__ void push_vertex_wave(const JobConst &c, JobState *s, const GraphDev 
     const int32_t d = g.beg[u + 1] - g.beg[u];
     nch = d < g.split ? 0 : (d + g.chunk - 1) / g.chunk;
   }
+  const int kind = nch > 0 ? 1 : 0;
+  const int key = li * 2 + kind;
+  const int slots = kind ? nch : 1;
   unsigned long long pend = __ballot(want);
+  int off = 0, tot = 0, leader = 0;
   while (pend) {
-    const int leader = __builtin_ctzll(pend);
-    const int l0 = __shfl(li, leader, 64);
-    const bool mine = ((pend >> lane) & 1) && li == l0;
-    const unsigned long long ms = __ballot(mine && nch == 0), mb = __ballot(mine && nch > 0);
-    if (ms) {
-      int base = 0;
-      if (lane == leader) base = atomicAdd(&s->cnt[l0][0], (int)__popcll(ms));
-      base = __shfl(base, leader, 64);
-      const int r = __builtin_amdgcn_mbcnt_hi((unsigned)(ms >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)ms, 0));
-      if (mine && nch == 0) c.lst[(size_t)l0 * g.n + base + r] = u;
-    }
-    if (mb) {
-      int x = (mine && nch > 0) ? nch : 0;
+    const int ld = __builtin_ctzll(pend);
+    const int k0 = __shfl(key, ld, 64);
+    const bool mine = ((pend >> lane) & 1) && key == k0;
+    int x = mine ? slots : 0;
 #pragma unroll
-      for (int off = 1; off < 64; off <<= 1) {
-        const int y = __shfl_up(x, off, 64);
-        if (lane >= off) x += y;
-      }
-      const int tot = __shfl(x, 63, 64);
-      int base = 0;
-      if (lane == leader) base = atomicAdd(&s->cnt[l0][1], tot);
-      base = __shfl(base, leader, 64);
-      if (mine && nch > 0) {
-        int32_t *b = c.blst + ((size_t)l0 * c.bcap + base + x - nch) * 2;
-        for (int i = 0; i < nch; i++) { b[2 * i] = u; b[2 * i + 1] = i; }
-      }
+    for (int o = 1; o < 64; o <<= 1) {
+      const int y = __shfl_up(x, o, 64);
+      if (lane >= o) x += y;
     }
-    pend &= ~(ms | mb);
+    const int t = __shfl(x, 63, 64);
+    if (mine) { off = x - slots; leader = ld; }
+    if (lane == ld) tot = t;
+    pend &= ~__ballot(mine);
+  }
+  int base = 0;
+  if (want && lane == leader) base = atomicAdd(&s->cnt[li][kind], tot);
+  base = __shfl(base, leader, 64);
+  if (!want) return;
+  if (!kind) {
+    c.lst[(size_t)li * g.n + base + off] = u;
+  } else {
+    int32_t *b = c.blst + ((size_t)li * c.bcap + base + off) * 2;
+    for (int i = 0; i < nch; i++) { b[2 * i] = u; b[2 * i + 1] = i; }
   }
 }
 
@@ -1108,23 +1106,44 @@ __global__ __launch_bounds__(BLK) void k_tables(const JobConst *jcs, JobState *j
 }
 
 // --------------------------------------------------------------------------- merge
-// Copies n consecutive words of src into LDS through put(i, value). Each lane issues
-// SU loads before its first LDS store, so a tile's rows arrive after a few memory
-// latencies rather than one latency per word.
+// Division by a uniform divisor dv >= 1: for dv >= 2, x / dv is the high word of
+// x * ceil(2^32 / dv), exact whenever x * dv < 2^32 (the product overshoots x / dv by less
+// than x / 2^32 < 1 / dv). Merge tiles use it for x < 2^16 and dv < 2^8.
+struct UDiv {
+  uint32_t d, m;
+  __device__ __forceinline__ explicit UDiv(uint32_t dv) : d(dv), m(dv > 1 ? 0xffffffffu / dv + 1u : 0u) {}
+  __device__ __forceinline__ uint32_t div(uint32_t x) const { return d == 1 ? x : __umulhi(x, m); }
+};
+
+// Stages levels 1..L-1 of the tile's vertices [v0, v0 + tv) into LDS as
+// rows[(vi * H + d) * W + p]. Each level is one contiguous segment of its level block
+// whose word i is row i / C(k,l), subset rank i % C(k,l). A lane issues SU loads before
+// its first LDS store.
 constexpr int SU = 8;
-template <class Put>
-__device__ __forceinline__ void stage_block(const int32_t *__restrict__ src, int n, Put put) {
-  for (int base = 0; base < n; base += SU * BLK) {
-    int32_t x[SU];
+__device__ __forceinline__ void stage_levels(const JobConst &c, int L, long long v0, int tv, int H,
+                                             int32_t *rows) {
+  const int W = c.W;
+  for (int l = 1; l < L; l++) {
+    const int ns = c.nsub[l];
+    const int n = tv * H * ns;
+    const UDiv dn(ns);
+    const int32_t *src = c.T + c.off[l] + v0 * H * ns;
+    const uint8_t *sub = c.sub[l];
+    for (int base = 0; base < n; base += SU * BLK) {
+      int32_t x[SU];
 #pragma unroll
-    for (int u = 0; u < SU; u++) {
-      const int i = base + u * BLK + (int)threadIdx.x;
-      x[u] = i < n ? src[i] : INF;
-    }
+      for (int u = 0; u < SU; u++) {
+        const int i = base + u * BLK + (int)threadIdx.x;
+        x[u] = i < n ? src[i] : INF;
+      }
 #pragma unroll
-    for (int u = 0; u < SU; u++) {
-      const int i = base + u * BLK + (int)threadIdx.x;
-      if (i < n) put(i, x[u]);
+      for (int u = 0; u < SU; u++) {
+        const int i = base + u * BLK + (int)threadIdx.x;
+        if (i < n) {
+          const int r = (int)dn.div((uint32_t)i);
+          rows[r * W + sub[i - r * ns]] = x[u];
+        }
+      }
     }
   }
 }
@@ -1179,19 +1198,13 @@ __global__ __launch_bounds__(BLK) void k_merge(const JobConst *jcs, JobState *js
     // every subset read below has fewer than L members, so staging overwrites all of it
     for (int i = threadIdx.x; i < tv; i += BLK) amask[i] = 0;
     if (threadIdx.x == 0) { ubs = load_ub(s); minc_s = INF; }
-    for (int l = 1; l < L; l++) {
-      const int ns = c.nsub[l];
-      const uint8_t *sub = c.sub[l];
-      stage_block(c.T + c.off[l] + (size_t)v0 * ns, tv * ns, [&](int i, int32_t x) {
-        const int vi = i / ns, r = i - vi * ns;
-        rows[vi * W + sub[r]] = x;
-      });
-    }
+    stage_levels(c, L, v0, tv, 1, rows);
     __syncthreads();
     const int ub = ubs;
     int ubl = ub, mact = INF;
+    const UDiv dnl(nL);
     for (int it = threadIdx.x; it < tv * nL; it += BLK) {
-      const int vi = it / nL, j = it - vi * nL;
+      const int vi = (int)dnl.div((uint32_t)it), j = it - vi * nL;
       const int p = c.sub[L][j];
       const int32_t *row = rows + vi * W;
       const int low = p & -p, rest = p ^ low;
@@ -1246,7 +1259,7 @@ __global__ __launch_bounds__(BLK) void k_merge(const JobConst *jcs, JobState *js
 // pm(d,q) = min_{d' <= d} T(v,q,d'), so a split costs two sums per target height nd:
 // d1 = nd with d2 <= min(nd, D - nd), and d2 = nd with d1 <= min(nd - 1, D - nd).
 constexpr int HMERGE_LDS_WORDS = 4096;
-constexpr int HMERGE_MASKS = HMERGE_LDS_WORDS / 4;   // tv * H <= 4096 / W, W >= 4
+constexpr int HMERGE_MASKS = HMERGE_LDS_WORDS / 8;   // tv * H <= 4096 / W for W >= 8; k = 2 tiles are clamped
 __global__ __launch_bounds__(BLK) void k_merge_hop(const JobConst *jcs, JobState *jss, const Plan *pl, GraphDev g) {
   __shared__ long long soff[MAXJOBS + 1];
   __shared__ int32_t rows[HMERGE_LDS_WORDS];
@@ -1272,18 +1285,10 @@ __global__ __launch_bounds__(BLK) void k_merge_hop(const JobConst *jcs, JobState
     const int tv = (int)min((long long)TV, (long long)g.n - v0);
     for (int i = threadIdx.x; i < tv * H; i += BLK) amask[i] = 0;
     // every subset read below has fewer than L members, so staging overwrites all of it
-    for (int l = 1; l < L; l++) {
-      const int ns = c.nsub[l], hn = H * ns;
-      const uint8_t *sub = c.sub[l];
-      stage_block(c.T + c.off[l] + (size_t)v0 * hn, tv * hn, [&](int i, int32_t x) {
-        const int vi = i / hn, rem = i - vi * hn;
-        const int d = rem / ns, r = rem - d * ns;
-        rows[vi * RW + d * W + sub[r]] = x;
-      });
-    }
+    stage_levels(c, L, v0, tv, H, rows);
     __syncthreads();
     for (int i = threadIdx.x; i < tv * W; i += BLK) {
-      const int vi = i / W, q = i - vi * W;
+      const int vi = i >> c.K, q = i & (W - 1);
       if (q == 0 || popc(q) >= L) continue;
       int m = INF;
       for (int d = 0; d < H; d++) {
@@ -1294,9 +1299,11 @@ __global__ __launch_bounds__(BLK) void k_merge_hop(const JobConst *jcs, JobState
     }
     __syncthreads();
     // item order (vertex, height, subset) makes the level-block write below contiguous
+    const UDiv dnl(nL), dh(H);
     for (int it = threadIdx.x; it < tv * nL * H; it += BLK) {
-      const int vi = it / (nL * H), rem = it - vi * nL * H;
-      const int nd = rem / nL, j = rem - nd * nL;
+      const int row = (int)dnl.div((uint32_t)it);   // vi * H + nd
+      const int j = it - row * nL;
+      const int vi = (int)dh.div((uint32_t)row), nd = row - vi * H;
       const int p = c.sub[L][j];
       const int low = p & -p, rest = p ^ low;
       const int l2 = min(nd, D - nd), l1 = min(nd - 1, D - nd);
@@ -1325,7 +1332,7 @@ __global__ __launch_bounds__(BLK) void k_merge_hop(const JobConst *jcs, JobState
     for (int b0 = 0; b0 < tv * H; b0 += BLK) {  // wave-uniform trip count for the aggregated push
       const int i = b0 + (int)threadIdx.x;
       const unsigned long long mk = i < tv * H ? amask[i] : 0ull;
-      const int vi = i / H, nd = i - vi * H;
+      const int vi = (int)dh.div((uint32_t)i), nd = i - vi * H;
       const int32_t v = (int32_t)(v0 + vi);
       if (mk) c.act[(size_t)nd * g.n + v] = mk;
       push_vertex_wave(c, s, g, nd, mk != 0, v);
