@@ -116,8 +116,19 @@ def calibrate_copy(mib):
     dst.copy_(src[3 * n:])       # source written last
     torch.cuda.synchronize()
     assert bool((dst[-4:] == 7).all())
+    # random gather: 2^22 int32 reads at distinct random 64-B sectors of the 4 * mib MiB
+    # source (one per sector at most), written densely
+    nsec = 4 * n // 16
+    g = 1 << 22
+    idx = (torch.randperm(nsec, device="cuda")[:g] * 16).to(torch.int64)
+    torch.cuda.synchronize()
+    out = torch.index_select(src, 0, idx)
+    torch.cuda.synchronize()
+    assert bool((out[:4] == 7).all())
     print(json.dumps({"calibrate_copy_mib": mib, "expected_kib_each_way": mib * 1024,
-                      "copies": ["cold source", "recently written source"]}))
+                      "copies": ["cold source", "recently written source"],
+                      "gather": {"reads": g, "distinct_64B_sectors": g, "index_kib": g * 8 // 1024,
+                                 "output_kib": g * 4 // 1024}}))
 
 
 def main():
@@ -127,6 +138,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--k", type=int, default=5)
+    ap.add_argument("--D", type=int, default=-1, help="diameter bound of the plain-path queries (-1: none)")
     ap.add_argument("--workload", default="cl3m")
     ap.add_argument("--plain-queries", type=int, default=24)
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU-baseline work")
@@ -186,20 +198,24 @@ def main():
     # first queries of the first timed batch so that both paths see the same queries
     plain = []
     plain_q = queries[args.warmup * args.batch:][:args.plain_queries]
-    eng.solve(plain_q[0], flags=flags)  # warm-up
+    eng.solve(plain_q[0], D=args.D, flags=flags)  # warm-up
     for q in plain_q:
-        plain.append(eng.solve(q, flags=flags))
+        plain.append(eng.solve(q, D=args.D, flags=flags))
     lat = np.array([r["t_total_s"] for r in plain]) * 1e3
     plain_qps = len(plain) / max(sum(r["t_total_s"] for r in plain), 1e-9)
     gb = sum(b_grow(r) for r in plain)
     gt = sum(r["t_grow_s"] for r in plain)
     grow_gbs = gb / gt / 1e9 if gt > 0 else 0.0
     grow_rounds = max(sum(r["grow_rounds"] for r in plain), 1)
+    tm = sum(r["t_merge_s"] for r in plain)
     print(f"# plain: {plain_qps:.2f} q/s p50 {np.median(lat):.2f} ms p99 {np.percentile(lat, 99):.2f} ms; "
-          f"grow {grow_gbs:.1f} GB/s ({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
+          f"grow {grow_gbs:.1f} GB/s over {grow_rounds} rounds ({gb / grow_rounds / 1e6:.1f} MB, "
+          f"{gt / grow_rounds * 1e6:.0f} us per round); merge {tm * 1e3:.1f} ms ({time.time() - t0:.1f}s)",
+          file=sys.stderr, flush=True)
     if args.profile_only:
         eng.close()
         return
+    assert args.D < 0, "the batched benchmark runs the non-hop workload; --D is for --profile-only"
 
     # ---- batched path: warmup then timed steps
     batches = [queries[i * args.batch:(i + 1) * args.batch] for i in range(args.warmup + args.steps)]
