@@ -593,7 +593,7 @@ int gst_solve(gst_ctx *c, const int32_t *gids, int32_t k, int32_t D, int32_t fla
         const int li = q.D >= 0 ? h.depth : h.parity;
         const int n0 = h.cnt[li][0], n1 = h.cnt[li][1];
         HIPCHK(hipEventRecord(c->ev0, c->st));
-        if (q.D < 0) k_select<<<grid_for(n0 + n1, BLK, GROW_GRID_CAP), BLK, 0, c->st>>>(c->d_jc, c->d_js, pl, g);
+        k_select<<<grid_for(n0 + n1, BLK, GROW_GRID_CAP), BLK, 0, c->st>>>(c->d_jc, c->d_js, pl, g);
         if (n0) k_grow_thread<<<grid_for(n0, BLK, GROW_GRID_CAP), BLK, 0, c->st>>>(c->d_jc, c->d_js, pl, g);
         if (n1) k_grow_wave<<<grid_for(n1, BLK / WAVE, GROW_GRID_CAP), BLK, 0, c->st>>>(c->d_jc, c->d_js, pl, g);
         HIPCHK(hipEventRecord(c->ev1, c->st));

@@ -461,6 +461,77 @@ __device__ __forceinline__ unsigned long long grow_select(const JobConst &c, Job
   return gm;
 }
 
+// Diameter-bounded variant: evaluate the active states of (v, level L, height d).
+// UB candidates (each is the value of a DP merge, so UB stays >= the DP optimum):
+// - the star of v: merging the singletons (v, {i}, h_i) with every h_i <= floor(D/2) is
+//   legal in any order (any two heights sum to <= D), so sum_i min_{h <= D/2} T(v,{i},h);
+// - a state with a final complement q = R \ p: T(v,p,d) + min_{d2 <= D-d} T(v,q,d2).
+// Lower bound (levels >= 2): on an optimal derivation through (v,p,d), the rest of the
+// derivation holds, for each group i outside p, a walk from v to a member of i of at
+// most D - d hops (up from v to the merge point, then down a branch; their heights add
+// up to <= D), so its cost is >= max_{i not in p} min_{h <= D-d} T(v,{i},h). Level 1 is
+// not pruned, so these distances are exact when levels >= 2 run. No UB/2 rule here: the
+// centroid re-rooting behind it changes heights.
+__device__ unsigned long long grow_select_hop(const JobConst &c, JobState *s, int32_t v,
+                                              unsigned long long a, int L, int d, int ub,
+                                              int &n_states) {
+  const int K = c.K, D = c.D, H = c.H;
+  const int32_t *l1 = lrow(c, 1, v, 0);  // H rows of K group distances, contiguous
+  int dlb[MAXK], dst[MAXK];
+#pragma unroll
+  for (int i = 0; i < MAXK; i++) { dlb[i] = INF; dst[i] = INF; }
+  for (int h = 0; h < H; h++) {
+#pragma unroll
+    for (int i = 0; i < MAXK; i++) {
+      if (i < K) {
+        const int x = l1[h * K + i];
+        if (h <= D - d && x < dlb[i]) dlb[i] = x;
+        if (2 * h <= D && x < dst[i]) dst[i] = x;
+      }
+    }
+  }
+  int ubl = ub;
+  {
+    long long star = 0;
+    bool ok = true;
+#pragma unroll
+    for (int i = 0; i < MAXK; i++)
+      if (i < K) { ok = ok && dst[i] < INF; star += dst[i]; }
+    if (ok && star < ubl) { atomicMin(&s->ub, (int)star); ubl = (int)star; }
+  }
+  const bool prune = !(c.flags & JF_NO_PRUNE) && L > 1;
+  const int32_t *row = lrow(c, L, v, d);
+  unsigned long long gm = 0;
+  while (a) {
+    const int j = __builtin_ctzll(a);
+    a &= a - 1;
+    n_states++;
+    const int p = c.sub[L][j];
+    const int cst = row[j];
+    if (cst >= INF) continue;
+    const int q = c.R ^ p;
+    if (q && popc(q) < L) {
+      const int Lq = popc(q);
+      const int32_t *cq = lrow(c, Lq, v, 0) + c.rank[q];
+      int mq = INF;
+      for (int d2 = 0; d2 <= D - d; d2++) {
+        const int x = cq[(size_t)d2 * c.nsub[Lq]];
+        mq = x < mq ? x : mq;
+      }
+      if (mq < INF && cst + mq < ubl) { atomicMin(&s->ub, cst + mq); ubl = cst + mq; }
+    }
+    if (prune) {
+      int lb = 0;
+#pragma unroll
+      for (int i = 0; i < MAXK; i++)
+        if (i < K && !((p >> i) & 1)) lb = dlb[i] > lb ? dlb[i] : lb;
+      if ((long long)cst + lb > ubl) continue;
+    }
+    gm |= 1ull << j;
+  }
+  return gm;
+}
+
 // Up to RG active subsets of one source vertex are relaxed together over each edge: the
 // RG target loads of an edge are issued back to back, an improvement is detected
 // against the loaded value, and the atomicMin is fire-and-forget (its result is unused,
@@ -752,19 +823,23 @@ __global__ __launch_bounds__(BLK) void k_select(const JobConst *jcs, JobState *j
         v = item[0];
         if (item[1] != 0) jb = -1;  // one evaluation per vertex
       }
-      if (jb >= 0 && jcs[jb].hop) jb = -1;
       if (jb >= 0) {
         if (jb != cj) { flush_lane(jcs, jss, cj, k); k.zero(); cj = jb; }
         const JobConst &c = jcs[jb];
         JobState *s = &jss[jb];
         unsigned long long *act = c.act + (size_t)cur_list(c, s) * g.n;
         const unsigned long long a = act[v];
-        unsigned long long defer = 0;
-        const unsigned long long gm =
-            grow_select(c, s, v, a, s->level, load_ub(s), k.states, s->theta, defer, k.dmin);
-        if (gm != a) act[v] = gm;
-        fl = FAR_LIST + s->fpar;
-        if (defer) want = atomicOr(&c.act[(size_t)FAR_ACT * g.n + v], defer) == 0;
+        if (c.hop) {
+          const unsigned long long gm = grow_select_hop(c, s, v, a, s->level, s->depth, load_ub(s), k.states);
+          if (gm != a) act[v] = gm;
+        } else {
+          unsigned long long defer = 0;
+          const unsigned long long gm =
+              grow_select(c, s, v, a, s->level, load_ub(s), k.states, s->theta, defer, k.dmin);
+          if (gm != a) act[v] = gm;
+          fl = FAR_LIST + s->fpar;
+          if (defer) want = atomicOr(&c.act[(size_t)FAR_ACT * g.n + v], defer) == 0;
+        }
       }
     }
     wp_push(pb, jcs, jss, g, jb, fl, want, v, 0);
@@ -901,9 +976,8 @@ __global__ __launch_bounds__(BLK) void k_grow_thread(const JobConst *jcs, JobSta
       dbase = lrow(c, L, 0, c.hop ? d + 1 : 0);
       act_lo = c.act + (size_t)lo * g.n;
       v = c.lst[(size_t)li * g.n + (it - soff[jb])];
-      gm = c.act[(size_t)li * g.n + v];  // already reduced by k_select (non-hop)
+      gm = c.act[(size_t)li * g.n + v];  // already reduced by k_select
       k.verts++;
-      if (c.hop) k.states += popc((uint32_t)gm) + popc((uint32_t)(gm >> 32));
       src = lrow(c, L, v, d);
       e0 = g.beg[v];
       deg = g.beg[v + 1] - e0;
@@ -979,12 +1053,9 @@ __global__ __launch_bounds__(BLK) void k_grow_wave(const JobConst *jcs, JobState
     unsigned long long *const act_lo = c.act + (size_t)lo * g.n;
     const int32_t *item = c.blst + ((size_t)li * c.bcap + (it - soff[jb])) * 2;
     const int32_t v = item[0], ch = item[1];
-    unsigned long long gm = c.act[(size_t)li * g.n + v];  // already reduced by k_select (non-hop)
+    unsigned long long gm = c.act[(size_t)li * g.n + v];  // already reduced by k_select
     int32_t *src = lrow(c, L, v, d);
-    if (lane == 0 && ch == 0) {
-      k.verts++;
-      if (c.hop) k.states += popc((uint32_t)gm) + popc((uint32_t)(gm >> 32));
-    }
+    if (lane == 0 && ch == 0) k.verts++;
     if (!gm) continue;
     const int nb = popc((uint32_t)gm) + popc((uint32_t)(gm >> 32));
     const int32_t eb = g.beg[v] + ch * g.chunk;
@@ -1265,6 +1336,8 @@ __global__ __launch_bounds__(BLK) void k_merge_hop(const JobConst *jcs, JobState
   __shared__ int32_t rows[HMERGE_LDS_WORDS];
   __shared__ int32_t pm[HMERGE_LDS_WORDS];
   __shared__ unsigned long long amask[HMERGE_MASKS];
+  __shared__ int ubs;
+  __shared__ int alive[256];  // tile vertex holds a finite state at a level in [2, L)
   const int nj = load_plan(pl, WK_MERGE, soff);
   const long long total = soff[nj];
   int cj = -1, minc_acc = INF;
@@ -1284,6 +1357,8 @@ __global__ __launch_bounds__(BLK) void k_merge_hop(const JobConst *jcs, JobState
     const long long v0 = (gt - soff[jb]) * TV;
     const int tv = (int)min((long long)TV, (long long)g.n - v0);
     for (int i = threadIdx.x; i < tv * H; i += BLK) amask[i] = 0;
+    if (threadIdx.x == 0) ubs = INF;
+    for (int i = threadIdx.x; i < tv; i += BLK) alive[i] = L == 2;
     // every subset read below has fewer than L members, so staging overwrites all of it
     stage_levels(c, L, v0, tv, H, rows);
     __syncthreads();
@@ -1296,14 +1371,28 @@ __global__ __launch_bounds__(BLK) void k_merge_hop(const JobConst *jcs, JobState
         m = x < m ? x : m;
         pm[vi * RW + d * W + q] = m;
       }
+      if (m < INF && popc(q) >= 2) alive[vi] = 1;
     }
     __syncthreads();
+    // UB candidates and the growth filter of the hop select pass (same bounds), here
+    // against the UB known when the tile starts
+    const int ub0 = load_ub(s);
+    const bool prune = !(c.flags & JF_NO_PRUNE);
+    int ubl = INF;
+    if (L == 2) {  // stars: arms of at most floor(D/2) hops, prefix minima of the singletons
+      for (int vi = threadIdx.x; vi < tv; vi += BLK) {
+        long long star = 0;
+        for (int i = 0; i < K; i++) star += pm[(vi * H + D / 2) * W + (1 << i)];
+        if (star < ubl) ubl = (int)star;
+      }
+    }
     // item order (vertex, height, subset) makes the level-block write below contiguous
     const UDiv dnl(nL), dh(H);
     for (int it = threadIdx.x; it < tv * nL * H; it += BLK) {
       const int row = (int)dnl.div((uint32_t)it);   // vi * H + nd
       const int j = it - row * nL;
       const int vi = (int)dh.div((uint32_t)row), nd = row - vi * H;
+      if (!alive[vi]) continue;  // only singletons: every split of p >= 3 members is INF
       const int p = c.sub[L][j];
       const int low = p & -p, rest = p ^ low;
       const int l2 = min(nd, D - nd), l1 = min(nd - 1, D - nd);
@@ -1324,11 +1413,25 @@ __global__ __launch_bounds__(BLK) void k_merge_hop(const JobConst *jcs, JobState
         if (sm == 0) break;
       }
       if (m >= INF) continue;
+      const int q = c.R ^ p;
+      const int32_t *pr = pm + (vi * H + (D - nd)) * W;  // minima over heights <= D - nd
+      if (q == 0) ubl = m < ubl ? m : ubl;
+      else if (popc(q) < L && (long long)m + pr[q] < ubl) ubl = m + pr[q];
+      if (prune && L < K) {
+        // m + lb > UB: no optimal derivation passes through this state, as a growth
+        // source or as a merge input, so it is not stored at all
+        int lb = 0;
+        for (int i = 0; i < K; i++)
+          if (!((p >> i) & 1)) lb = pr[1 << i] > lb ? pr[1 << i] : lb;
+        if ((long long)m + lb > ub0) continue;
+      }
       merged++;
       c.T[(size_t)c.off[L] + (size_t)v0 * H * nL + it] = m;
       if (L < K && nd < D) atomicOr(&amask[vi * H + nd], 1ull << j);
     }
+    if (ubl < ub0) atomicMin(&ubs, ubl);
     __syncthreads();
+    if (threadIdx.x == 0 && ubs < ub0) atomicMin(&s->ub, ubs);
     for (int b0 = 0; b0 < tv * H; b0 += BLK) {  // wave-uniform trip count for the aggregated push
       const int i = b0 + (int)threadIdx.x;
       const unsigned long long mk = i < tv * H ? amask[i] : 0ull;

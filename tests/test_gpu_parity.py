@@ -95,3 +95,20 @@ def test_caller_order_matches_renumbered(gpu_engine, rmat, rmat_inst):
                     assert r["root"] in set(r["edges"][:, :2].ravel().tolist()), (q, D, r["root"])
     finally:
         keep.close()
+
+
+@pytest.mark.parametrize("D", [1, 2, 3, 5])
+def test_rmat_hop_pruned_matches_unpruned(gpu_engine, rmat, D):
+    """The diameter-bounded search prunes by a lower bound against an upper bound; with
+    GST_F_NO_PRUNE it evaluates every state. Both give the same optimum."""
+    rng = np.random.default_rng(300 + D)
+    for i in range(5):
+        k = int(rng.integers(2, 6))
+        q = rng.choice(rmat.groups.n_groups, k, replace=False).astype(np.int32)
+        print("hop prune D", D, "query", i, q, flush=True)
+        a = gpu_engine.solve(q, D=D)
+        b = gpu_engine.solve(q, D=D, flags=engine.F_NO_PRUNE)
+        assert a["status"] == b["status"] and a["cost"] == b["cost"], (q, D, a["cost"], b["cost"])
+        if a["status"] == engine.GST_OK:
+            ok, why = engine.validate_tree(rmat.graph, rmat.groups, q, a["edges"], a["cost"], D)
+            assert ok, (q, D, why)
