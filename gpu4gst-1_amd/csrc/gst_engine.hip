@@ -18,6 +18,7 @@
 #include <cstring>
 #include <deque>
 #include <map>
+#include <list>
 #include <memory>
 #include <mutex>
 #include <set>
@@ -90,9 +91,12 @@ struct gst_ctx {
   long long bcap = 0;  // wavefront items one worklist can hold
   gst_opts opts{};
   int32_t split = 256, chunk = 4096, delta = 0;
-  // single-query path
+  // single-query path: one slot per (K, D) shape, least recently used evicted beyond
+  // plain_cap bytes (a mixed stream of shapes would otherwise reallocate its arena,
+  // up to V * 2^k * (D+1) * 4 bytes, on almost every query)
   std::mutex solve_mu;
-  Slot plain;
+  std::list<std::pair<std::pair<int, int>, Slot>> plain;  // front = most recent
+  int64_t plain_cap = 0;
   JobConst *d_jc = nullptr;  // [max_batch]
   JobState *d_js = nullptr;  // [max_batch]
   JobState *h_js = nullptr;  // pinned mirror [max_batch]
@@ -173,6 +177,34 @@ int slot_alloc(gst_ctx *c, Slot &s, int K, int D) {
     s.out = (int32_t *)take((size_t)ECAP * 12);
     s.stx = (unsigned long long *)take((size_t)NSTRIPE * NSTX * 8);
   }
+  return GST_OK;
+}
+
+// Bring the single-query slot of shape (K, D) to the front of the cache, allocating it
+// (and evicting the least recently used slots beyond plain_cap) when absent.
+int plain_slot(gst_ctx *c, int K, int D) {
+  for (auto it = c->plain.begin(); it != c->plain.end(); ++it)
+    if (it->first == std::make_pair(K, D)) {
+      c->plain.splice(c->plain.begin(), c->plain, it);
+      return GST_OK;
+    }
+  const int64_t need = (int64_t)slot_bytes(c, K, D);
+  int64_t held = 0;
+  for (auto &e : c->plain) held += (int64_t)e.second.bytes;
+  while (!c->plain.empty() && held + need > c->plain_cap) {
+    held -= (int64_t)c->plain.back().second.bytes;
+    slot_free(c->plain.back().second);
+    c->plain.pop_back();
+  }
+  Slot s;
+  int rc = slot_alloc(c, s, K, D);
+  if (rc != GST_OK && !c->plain.empty()) {  // device memory short: drop the cache, retry once
+    for (auto &e : c->plain) slot_free(e.second);
+    c->plain.clear();
+    rc = slot_alloc(c, s, K, D);
+  }
+  if (rc != GST_OK) return rc;
+  c->plain.emplace_front(std::make_pair(K, D), s);
   return GST_OK;
 }
 
@@ -503,6 +535,11 @@ int gst_open(const gst_csr *csr, const gst_groups *groups, const gst_opts *opts,
   HIPCHK(hipEventCreate(&c->ev1));
   c->g.beg = c->d_beg; c->g.adj = c->d_adj; c->g.w = c->d_w;
   c->g.goffs = c->d_goffs; c->g.gverts = c->d_gverts; c->g.nchunk = c->d_nchunk; c->g.n = n;
+  {
+    size_t fr = 0, tot = 0;
+    HIPCHK(hipMemGetInfo(&fr, &tot));
+    c->plain_cap = std::min<int64_t>((int64_t)fr / 8, (int64_t)32 << 30);
+  }
   int rc = gst_set_tuning(c.get(), c->split, c->chunk, c->delta);
   if (rc) return rc;
   rc = batch_open(c.get());
@@ -526,7 +563,8 @@ int gst_set_tuning(gst_ctx *c, int32_t degree_split, int32_t chunk_edges, int32_
   bcap = std::max<long long>(bcap, 1);
   if (bcap > c->bcap) {
     // slots sized for the old capacity must be rebuilt
-    slot_free(c->plain);
+    for (auto &e : c->plain) slot_free(e.second);
+    c->plain.clear();
     batch_release_slots(c);
   }
   c->bcap = std::max(c->bcap, bcap);
@@ -546,12 +584,9 @@ int gst_solve(gst_ctx *c, const int32_t *gids, int32_t k, int32_t D, int32_t fla
   int rc = check_query(c, gids, k, D, flags, q);
   if (rc) { out->status = rc; out->cost = -1; return rc; }
   HIPCHK(hipSetDevice(c->dev));
-  Slot &s = c->plain;
-  if (!s.T || s.K != q.K || s.D != q.D) {
-    // reuse a larger-or-equal slot only when the shape matches exactly; else reallocate
-    rc = slot_alloc(c, s, q.K, q.D);
-    if (rc) { out->status = rc; out->cost = -1; return rc; }
-  }
+  rc = plain_slot(c, q.K, q.D);
+  if (rc) { out->status = rc; out->cost = -1; return rc; }
+  Slot &s = c->plain.front().second;
   JobConst jc;
   fill_jobconst(c, s, q, jc);
   JobState js;
@@ -677,7 +712,8 @@ void gst_close(gst_ctx *c) {
   if (!c) return;
   (void)hipSetDevice(c->dev);
   batch_close(c);
-  slot_free(c->plain);
+  for (auto &e : c->plain) slot_free(e.second);
+  c->plain.clear();
   (void)hipFree(c->d_beg); (void)hipFree(c->d_adj); (void)hipFree(c->d_w);
   (void)hipFree(c->d_goffs); (void)hipFree(c->d_gverts); (void)hipFree(c->d_nchunk);
   (void)hipFree(c->d_jc); (void)hipFree(c->d_js); (void)hipHostFree(c->h_js);

@@ -1188,35 +1188,50 @@ struct UDiv {
 
 // Stages levels 1..L-1 of the tile's vertices [v0, v0 + tv) into LDS as
 // rows[(vi * H + d) * W + p]. Each level is one contiguous segment of its level block
-// whose word i is row i / C(k,l), subset rank i % C(k,l). A lane issues SU loads before
-// its first LDS store.
+// whose word i is row i / C(k,l), subset rank i % C(k,l); a segment holds
+// tv * H * C(k,l) <= 4096 * C(k,l) / W <= 2048 = SU * BLK words (C(k,l) <= 2^(k-1)), so
+// every lane issues the loads of all levels (at most (MAXK-1) * SU) before the first LDS
+// store: one memory latency per tile. lsub is the job's subset table, copied to LDS.
 constexpr int SU = 8;
 __device__ __forceinline__ void stage_levels(const JobConst &c, int L, long long v0, int tv, int H,
-                                             int32_t *rows) {
+                                             const uint8_t (*lsub)[64], int32_t *rows) {
   const int W = c.W;
-  for (int l = 1; l < L; l++) {
-    const int ns = c.nsub[l];
-    const int n = tv * H * ns;
-    const UDiv dn(ns);
-    const int32_t *src = c.T + c.off[l] + v0 * H * ns;
-    const uint8_t *sub = c.sub[l];
-    for (int base = 0; base < n; base += SU * BLK) {
-      int32_t x[SU];
+  const int t = (int)threadIdx.x;
+  int32_t x[MAXK - 1][SU];
+#pragma unroll
+  for (int l = 1; l < MAXK; l++) {
+    if (l < L) {
+      const int ns = c.nsub[l], n = tv * H * ns;
+      const int32_t *src = c.T + c.off[l] + v0 * H * ns;
 #pragma unroll
       for (int u = 0; u < SU; u++) {
-        const int i = base + u * BLK + (int)threadIdx.x;
-        x[u] = i < n ? src[i] : INF;
+        const int i = u * BLK + t;
+        x[l - 1][u] = i < n ? src[i] : INF;
       }
+    }
+  }
+  __syncthreads();  // lsub (written before the call) is visible from here
+#pragma unroll
+  for (int l = 1; l < MAXK; l++) {
+    if (l < L) {
+      const int ns = c.nsub[l], n = tv * H * ns;
+      const UDiv dn(ns);
 #pragma unroll
       for (int u = 0; u < SU; u++) {
-        const int i = base + u * BLK + (int)threadIdx.x;
+        const int i = u * BLK + t;
         if (i < n) {
           const int r = (int)dn.div((uint32_t)i);
-          rows[r * W + sub[i - r * ns]] = x[u];
+          rows[r * W + lsub[l][i - r * ns]] = x[l - 1][u];
         }
       }
     }
   }
+}
+
+// Copies the job's subset table (8 x 64 bytes) to LDS; visible after the next barrier
+// (stage_levels holds one).
+__device__ __forceinline__ void load_lsub(const JobConst &c, uint8_t (*lsub)[64]) {
+  if (threadIdx.x < 128) ((uint32_t *)lsub)[threadIdx.x] = ((const uint32_t *)c.sub)[threadIdx.x];
 }
 
 // Merge counters are kept per thread across the tiles of one job and flushed when the
@@ -1243,6 +1258,8 @@ __global__ __launch_bounds__(BLK) void k_merge(const JobConst *jcs, JobState *js
   __shared__ int32_t rows[MERGE_LDS_WORDS];
   __shared__ unsigned long long amask[512];
   __shared__ int ubs, minc_s;
+  __shared__ uint32_t lsubw[128];
+  uint8_t (*lsub)[64] = (uint8_t (*)[64])lsubw;
   const int nj = load_plan(pl, WK_MERGE, soff);
   const long long total = soff[nj];
   int cj = -1, minc_acc = INF;
@@ -1269,14 +1286,15 @@ __global__ __launch_bounds__(BLK) void k_merge(const JobConst *jcs, JobState *js
     // every subset read below has fewer than L members, so staging overwrites all of it
     for (int i = threadIdx.x; i < tv; i += BLK) amask[i] = 0;
     if (threadIdx.x == 0) { ubs = load_ub(s); minc_s = INF; }
-    stage_levels(c, L, v0, tv, 1, rows);
+    load_lsub(c, lsub);
+    stage_levels(c, L, v0, tv, 1, lsub, rows);
     __syncthreads();
     const int ub = ubs;
     int ubl = ub, mact = INF;
     const UDiv dnl(nL);
     for (int it = threadIdx.x; it < tv * nL; it += BLK) {
       const int vi = (int)dnl.div((uint32_t)it), j = it - vi * nL;
-      const int p = c.sub[L][j];
+      const int p = lsub[L][j];
       const int32_t *row = rows + vi * W;
       const int low = p & -p, rest = p ^ low;
       int m = INF;
@@ -1337,7 +1355,9 @@ __global__ __launch_bounds__(BLK) void k_merge_hop(const JobConst *jcs, JobState
   __shared__ int32_t pm[HMERGE_LDS_WORDS];
   __shared__ unsigned long long amask[HMERGE_MASKS];
   __shared__ int ubs;
-  __shared__ int alive[256];  // tile vertex holds a finite state at a level in [2, L)
+  __shared__ int alive[256];
+  __shared__ uint32_t lsubw[128];
+  uint8_t (*lsub)[64] = (uint8_t (*)[64])lsubw;  // tile vertex holds a finite state at a level in [2, L)
   const int nj = load_plan(pl, WK_MERGE, soff);
   const long long total = soff[nj];
   int cj = -1, minc_acc = INF;
@@ -1360,7 +1380,8 @@ __global__ __launch_bounds__(BLK) void k_merge_hop(const JobConst *jcs, JobState
     if (threadIdx.x == 0) ubs = INF;
     for (int i = threadIdx.x; i < tv; i += BLK) alive[i] = L == 2;
     // every subset read below has fewer than L members, so staging overwrites all of it
-    stage_levels(c, L, v0, tv, H, rows);
+    load_lsub(c, lsub);
+    stage_levels(c, L, v0, tv, H, lsub, rows);
     __syncthreads();
     for (int i = threadIdx.x; i < tv * W; i += BLK) {
       const int vi = i >> c.K, q = i & (W - 1);
@@ -1393,7 +1414,7 @@ __global__ __launch_bounds__(BLK) void k_merge_hop(const JobConst *jcs, JobState
       const int j = it - row * nL;
       const int vi = (int)dh.div((uint32_t)row), nd = row - vi * H;
       if (!alive[vi]) continue;  // only singletons: every split of p >= 3 members is INF
-      const int p = c.sub[L][j];
+      const int p = lsub[L][j];
       const int low = p & -p, rest = p ^ low;
       const int l2 = min(nd, D - nd), l1 = min(nd - 1, D - nd);
       const int32_t *rn = rows + vi * RW + nd * W;
