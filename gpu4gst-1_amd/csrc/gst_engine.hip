@@ -652,6 +652,10 @@ int gst_solve(gst_ctx *c, const int32_t *gids, int32_t k, int32_t D, int32_t fla
         HIPCHK(hipEventRecord(c->ev0, c->st));
         if (q.D >= 0) k_merge_hop<<<grid_for(tiles, 1, merge_grid), BLK, 0, c->st>>>(c->d_jc, c->d_js, pl, g);
         else k_merge<<<grid_for(tiles, 1, merge_grid), BLK, 0, c->st>>>(c->d_jc, c->d_js, pl, g);
+        if (h.level < q.K && (q.D != 0)) {
+          const long long nblk = ((long long)c->n + 63) / 64 * (q.D > 0 ? q.D : 1);
+          k_collect<<<grid_for(nblk, BLK / WAVE, GROW_GRID_CAP), BLK, 0, c->st>>>(c->d_jc, c->d_js, pl, g);
+        }
         HIPCHK(hipEventRecord(c->ev1, c->st));
         timed_phase = PH_MERGE;
         break;

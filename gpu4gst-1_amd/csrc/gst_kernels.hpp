@@ -117,7 +117,8 @@ constexpr int FAR_LIST = 2;
 constexpr int FAR_ACT = 2;
 
 // Work items of one round, flattened over jobs: off[kind][j] = first item of job j.
-enum WorkKind : int32_t { WK_SEED = 0, WK_THR = 1, WK_WAVE = 2, WK_CLEAR = 3, WK_MERGE = 4, WK_FINAL = 5, WK_FAR = 6, NWK = 7 };
+enum WorkKind : int32_t { WK_SEED = 0, WK_THR = 1, WK_WAVE = 2, WK_CLEAR = 3, WK_MERGE = 4, WK_FINAL = 5, WK_FAR = 6,
+                          WK_COLLECT = 7, NWK = 8 };
 struct Plan {
   int32_t njobs, pad;
   long long off[NWK][MAXJOBS + 1];
@@ -329,7 +330,7 @@ __global__ __launch_bounds__(BLK) void k_step(const JobConst *jcs, JobState *jss
     JobState &s = jss[j];
     step_job(c, s);
     const int ph = s.phase;
-    long long k[NWK] = {0, 0, 0, 0, 0, 0, 0};
+    long long k[NWK] = {0, 0, 0, 0, 0, 0, 0, 0};
     if (ph == PH_SEED) k[WK_SEED] = c.seed_total;
     if (ph == PH_GROW) {
       const int li = c.hop ? s.depth : s.parity;
@@ -339,6 +340,8 @@ __global__ __launch_bounds__(BLK) void k_step(const JobConst *jcs, JobState *jss
     }
     if (ph == PH_HOPCLEAR) k[WK_CLEAR] = s.cnt[c.D][0] + s.cnt[c.D][1];
     if (ph == PH_MERGE) k[WK_MERGE] = ((long long)n + c.tv - 1) / c.tv;
+    // blocks of 64 vertices per list the merge activates into (heights 0..D-1, or one list)
+    if (ph == PH_MERGE && s.level < c.K) k[WK_COLLECT] = (((long long)n + 63) / 64) * (c.hop ? c.D : 1);
     if (ph == PH_FINAL) k[WK_FINAL] = ((long long)n * c.H + FINAL_CHUNK - 1) / FINAL_CHUNK;
     if (ph == PH_FARSPLIT) k[WK_FAR] = s.cnt[FAR_LIST + s.fpar][0];
 #pragma unroll
@@ -739,50 +742,6 @@ __device__ __forceinline__ void wp_push(WavePush &b, const JobConst *jcs, JobSta
   }
   b.ns += cs;
   b.nb += cb;
-}
-
-// push_vertex for every lane with want, li may differ between lanes. Lanes are grouped
-// by (list, kind) with ballots and scans only; then every group's lowest lane reserves
-// the group's range with one atomicAdd, all groups in the same instruction, so the wave
-// waits for one memory round trip. Called converged.
-__device__ void push_vertex_wave(const JobConst &c, JobState *s, const GraphDev &g, int li,
-                                 bool want, int32_t u) {
-  const int lane = lane_id();
-  int nch = 0;
-  if (want) {
-    const int32_t d = g.beg[u + 1] - g.beg[u];
-    nch = d < g.split ? 0 : (d + g.chunk - 1) / g.chunk;
-  }
-  const int kind = nch > 0 ? 1 : 0;
-  const int key = li * 2 + kind;
-  const int slots = kind ? nch : 1;
-  unsigned long long pend = __ballot(want);
-  int off = 0, tot = 0, leader = 0;
-  while (pend) {
-    const int ld = __builtin_ctzll(pend);
-    const int k0 = __shfl(key, ld, 64);
-    const bool mine = ((pend >> lane) & 1) && key == k0;
-    int x = mine ? slots : 0;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      const int y = __shfl_up(x, o, 64);
-      if (lane >= o) x += y;
-    }
-    const int t = __shfl(x, 63, 64);
-    if (mine) { off = x - slots; leader = ld; }
-    if (lane == ld) tot = t;
-    pend &= ~__ballot(mine);
-  }
-  int base = 0;
-  if (want && lane == leader) base = atomicAdd(&s->cnt[li][kind], tot);
-  base = __shfl(base, leader, 64);
-  if (!want) return;
-  if (!kind) {
-    c.lst[(size_t)li * g.n + base + off] = u;
-  } else {
-    int32_t *b = c.blst + ((size_t)li * c.bcap + base + off) * 2;
-    for (int i = 0; i < nch; i++) { b[2 * i] = u; b[2 * i + 1] = i; }
-  }
 }
 
 // --------------------------------------------------------------------------- select
@@ -1329,13 +1288,8 @@ __global__ __launch_bounds__(BLK) void k_merge(const JobConst *jcs, JobState *js
     __syncthreads();
     if (threadIdx.x == 0 && ubs < ub) atomicMin(&s->ub, ubs);
     if (threadIdx.x == 0) minc_acc = minc_s < minc_acc ? minc_s : minc_acc;
-    for (int b0 = 0; b0 < tv; b0 += BLK) {  // wave-uniform trip count for the aggregated push
-      const int vi = b0 + (int)threadIdx.x;
-      const unsigned long long mk = vi < tv ? amask[vi] : 0ull;
-      const int32_t v = (int32_t)(v0 + vi);
-      if (mk) c.act[(size_t)lo * g.n + v] = mk;
-      push_vertex_wave(c, s, g, lo, mk != 0, v);
-    }
+    for (int vi = threadIdx.x; vi < tv; vi += BLK)  // k_collect builds the worklist
+      if (amask[vi]) c.act[(size_t)lo * g.n + v0 + vi] = amask[vi];
     __syncthreads();
   }
   merged_acc += merged;
@@ -1343,21 +1297,26 @@ __global__ __launch_bounds__(BLK) void k_merge(const JobConst *jcs, JobState *js
   flush_merge(jcs, jss, cj, merged_acc, minc_acc);
 }
 
-// Hop merge for level L: T(v,p,max(d1,d2)) = min T(v,p1,d1) + T(v,p2,d2), d1 + d2 <= D.
-// Next to the staged rows the tile keeps their prefix minima over height,
-// pm(d,q) = min_{d' <= d} T(v,q,d'), so a split costs two sums per target height nd:
-// d1 = nd with d2 <= min(nd, D - nd), and d2 = nd with d1 <= min(nd - 1, D - nd).
+// Hop merge for level L. A stored hop state (v,p,d) holds the cost of a tree rooted at v
+// covering p whose height is at most d (a grow step gives height exactly d; a merge may
+// store a lower tree at d). Every later use only needs "at most": a merge of heights
+// <= d1 and <= d2 with d1 + d2 <= D has diameter <= D, and a grow adds one. So the merge
+// takes, over the staged prefix minima pm(h,q) = min_{h' <= h} T(v,q,h'),
+//   T(v,p,nd) = min over splits p1 + p2 = p of
+//               min(pm(nd,p1) + pm(b,p2), pm(b,p1) + pm(nd,p2)),  b = min(nd, D - nd),
+// the best pair of heights (d1, d2) with max(d1, d2) <= nd and d1 + d2 <= D. The minimum
+// over heights of the full states, the answer, is the same as with exact heights.
+// Splits whose halves have no finite state at the vertex are skipped (finm).
 constexpr int HMERGE_LDS_WORDS = 4096;
 constexpr int HMERGE_MASKS = HMERGE_LDS_WORDS / 8;   // tv * H <= 4096 / W for W >= 8; k = 2 tiles are clamped
 __global__ __launch_bounds__(BLK) void k_merge_hop(const JobConst *jcs, JobState *jss, const Plan *pl, GraphDev g) {
   __shared__ long long soff[MAXJOBS + 1];
-  __shared__ int32_t rows[HMERGE_LDS_WORDS];
   __shared__ int32_t pm[HMERGE_LDS_WORDS];
   __shared__ unsigned long long amask[HMERGE_MASKS];
+  __shared__ unsigned long long finm[256][2];  // per tile vertex: subsets with a finite state
   __shared__ int ubs;
-  __shared__ int alive[256];
   __shared__ uint32_t lsubw[128];
-  uint8_t (*lsub)[64] = (uint8_t (*)[64])lsubw;  // tile vertex holds a finite state at a level in [2, L)
+  uint8_t (*lsub)[64] = (uint8_t (*)[64])lsubw;
   const int nj = load_plan(pl, WK_MERGE, soff);
   const long long total = soff[nj];
   int cj = -1, minc_acc = INF;
@@ -1377,27 +1336,27 @@ __global__ __launch_bounds__(BLK) void k_merge_hop(const JobConst *jcs, JobState
     const long long v0 = (gt - soff[jb]) * TV;
     const int tv = (int)min((long long)TV, (long long)g.n - v0);
     for (int i = threadIdx.x; i < tv * H; i += BLK) amask[i] = 0;
+    for (int i = threadIdx.x; i < 2 * tv; i += BLK) finm[i >> 1][i & 1] = 0;
     if (threadIdx.x == 0) ubs = INF;
-    for (int i = threadIdx.x; i < tv; i += BLK) alive[i] = L == 2;
+    // UB known when the tile starts (its load overlaps the staging): UB candidates of the
+    // tile are compared with it, and the growth filter of the hop select pass uses it
+    const int ub0 = load_ub(s);
     // every subset read below has fewer than L members, so staging overwrites all of it
     load_lsub(c, lsub);
-    stage_levels(c, L, v0, tv, H, lsub, rows);
+    stage_levels(c, L, v0, tv, H, lsub, pm);
     __syncthreads();
-    for (int i = threadIdx.x; i < tv * W; i += BLK) {
+    for (int i = threadIdx.x; i < tv * W; i += BLK) {  // prefix minima over height, in place
       const int vi = i >> c.K, q = i & (W - 1);
       if (q == 0 || popc(q) >= L) continue;
       int m = INF;
       for (int d = 0; d < H; d++) {
-        const int x = rows[vi * RW + d * W + q];
-        m = x < m ? x : m;
-        pm[vi * RW + d * W + q] = m;
+        int32_t *x = &pm[vi * RW + d * W + q];
+        m = *x < m ? *x : m;
+        *x = m;
       }
-      if (m < INF && popc(q) >= 2) alive[vi] = 1;
+      if (m < INF) atomicOr(&finm[vi][q >> 6], 1ull << (q & 63));
     }
     __syncthreads();
-    // UB candidates and the growth filter of the hop select pass (same bounds), here
-    // against the UB known when the tile starts
-    const int ub0 = load_ub(s);
     const bool prune = !(c.flags & JF_NO_PRUNE);
     int ubl = INF;
     if (L == 2) {  // stars: arms of at most floor(D/2) hops, prefix minima of the singletons
@@ -1413,21 +1372,22 @@ __global__ __launch_bounds__(BLK) void k_merge_hop(const JobConst *jcs, JobState
       const int row = (int)dnl.div((uint32_t)it);   // vi * H + nd
       const int j = it - row * nL;
       const int vi = (int)dh.div((uint32_t)row), nd = row - vi * H;
-      if (!alive[vi]) continue;  // only singletons: every split of p >= 3 members is INF
       const int p = lsub[L][j];
+      const unsigned long long f0 = finm[vi][0], f1 = finm[vi][1];
       const int low = p & -p, rest = p ^ low;
-      const int l2 = min(nd, D - nd), l1 = min(nd - 1, D - nd);
-      const int32_t *rn = rows + vi * RW + nd * W;
-      const int32_t *q2 = pm + vi * RW + l2 * W;
-      const int32_t *q1 = pm + vi * RW + max(l1, 0) * W;
+      const int b = min(nd, D - nd);
+      const int32_t *pn = pm + vi * RW + nd * W;  // heights <= nd
+      const int32_t *pb = pm + vi * RW + b * W;   // heights <= b
       int m = INF;
       for (int sm = rest;; sm = (sm - 1) & rest) {
         const int p1 = low | sm, p2 = p ^ p1;
         if (p1 != p) {
-          const int t = rn[p1] + q2[p2];
-          m = t < m ? t : m;
-          if (l1 >= 0) {
-            const int u = q1[p1] + rn[p2];
+          const bool l1 = ((p1 < 64 ? f0 >> p1 : f1 >> (p1 - 64)) & 1) != 0;
+          const bool l2 = ((p2 < 64 ? f0 >> p2 : f1 >> (p2 - 64)) & 1) != 0;
+          if (l1 && l2) {
+            const int t = pn[p1] + pb[p2];
+            const int u = pb[p1] + pn[p2];
+            m = t < m ? t : m;
             m = u < m ? u : m;
           }
         }
@@ -1453,19 +1413,49 @@ __global__ __launch_bounds__(BLK) void k_merge_hop(const JobConst *jcs, JobState
     if (ubl < ub0) atomicMin(&ubs, ubl);
     __syncthreads();
     if (threadIdx.x == 0 && ubs < ub0) atomicMin(&s->ub, ubs);
-    for (int b0 = 0; b0 < tv * H; b0 += BLK) {  // wave-uniform trip count for the aggregated push
-      const int i = b0 + (int)threadIdx.x;
-      const unsigned long long mk = i < tv * H ? amask[i] : 0ull;
-      const int vi = (int)dh.div((uint32_t)i), nd = i - vi * H;
-      const int32_t v = (int32_t)(v0 + vi);
-      if (mk) c.act[(size_t)nd * g.n + v] = mk;
-      push_vertex_wave(c, s, g, nd, mk != 0, v);
+    for (int i = threadIdx.x; i < tv * H; i += BLK) {  // k_collect builds the worklists
+      const unsigned long long mk = amask[i];
+      if (mk) {
+        const int vi = (int)dh.div((uint32_t)i), nd = i - vi * H;
+        c.act[(size_t)nd * g.n + v0 + vi] = mk;
+      }
     }
     __syncthreads();
   }
   merged_acc += merged;
   }
   flush_merge(jcs, jss, cj, merged_acc, minc_acc);
+}
+
+// Worklists of the vertices a merge activated. The merge kernels write the activation
+// masks only; this pass scans them, 64 vertices of one list per wavefront item, and
+// pushes every vertex with a nonzero mask through the wavefront push buffer, so the
+// list-counter round trips are off the merge tiles' critical path. Masks outside the
+// lists are zero when a merge starts (consumed masks are cleared every round), so a
+// nonzero mask is exactly a merge activation.
+__global__ __launch_bounds__(BLK) void k_collect(const JobConst *jcs, JobState *jss, const Plan *pl, GraphDev g) {
+  __shared__ long long soff[MAXJOBS + 1];
+  __shared__ int32_t pb_small[(BLK / WAVE) * PB_SMALL];
+  __shared__ int32_t pb_big[(BLK / WAVE) * 2 * PB_BIG];
+  WavePush pb;
+  wp_init(pb, pb_small, pb_big);
+  const int nj = load_plan(pl, WK_COLLECT, soff);
+  const long long total = soff[nj];
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const long long nblk = ((long long)g.n + 63) / 64;
+  const long long waves_per_grid = (long long)gridDim.x * (BLK / WAVE);
+  for (long long it = (long long)blockIdx.x * (BLK / WAVE) + wave; it < total; it += waves_per_grid) {
+    const int jb = find_job(soff, nj, it);
+    const JobConst &c = jcs[jb];
+    const JobState *s = &jss[jb];
+    const long long r = it - soff[jb];
+    const int li = c.hop ? (int)(r / nblk) : (s->parity ^ 1);
+    const long long v = (c.hop ? r % nblk : r) * 64 + lane_id();
+    const bool want = v < g.n && c.act[(size_t)li * g.n + v] != 0;
+    const int nch = want ? g.nchunk[v] : 0;
+    wp_push(pb, jcs, jss, g, jb, li, want, (int32_t)v, nch);
+  }
+  wp_flush(pb, jcs, jss, g);
 }
 
 // --------------------------------------------------------------------------- final
@@ -1568,7 +1558,7 @@ __global__ __launch_bounds__(RB_BLK) void k_rebuild(const JobConst *jcs, JobStat
       const int p1 = low | sm;
       if (p1 == p) continue;
       if (c.hop) {
-        if ((d1 > d2 ? d1 : d2) != d || d1 + d2 > D) continue;
+        if ((d1 > d2 ? d1 : d2) > d || d1 + d2 > D) continue;  // merged hop states: height <= d
       } else if (d1 | d2) continue;
       const long long t = (long long)c.T[sidx(c, v, p1, d1)] + c.T[sidx(c, v, p ^ p1, d2)];
       if (t <= cost) atomicMin((unsigned long long *)&found, (unsigned long long)it);
