@@ -572,6 +572,16 @@ int gst_set_tuning(gst_ctx *c, int32_t degree_split, int32_t chunk_edges, int32_
   c->chunk = chunk_edges;
   c->g.split = degree_split;
   c->g.chunk = chunk_edges;
+  // with vertices renumbered by descending degree the wavefront vertices are a prefix
+  c->g.n_heavy = c->n;
+  if (!c->old_of_new.empty()) {
+    int32_t h = 0;
+    while (h < c->n && c->h_deg[h] >= degree_split) h++;
+    bool prefix = true;
+    for (int32_t v = h; v < c->n && prefix; v++) prefix = c->h_deg[v] < degree_split;
+    if (prefix) c->g.n_heavy = h;
+  }
+  batch_drop_graph(c);
   return GST_OK;
 }
 

@@ -134,8 +134,14 @@ struct GraphDev {
   int32_t n;
   int32_t split;        // degree split thread/wavefront
   int32_t chunk;        // edges per wavefront item
-  int32_t pad;
+  int32_t n_heavy;      // vertices >= n_heavy have deg < split (nchunk 0); n when unknown
 };
+
+// nchunk[u] without a memory access for light vertices: with vertices numbered by
+// descending degree, those of degree >= split are exactly the ids below n_heavy.
+__device__ __forceinline__ int32_t chunks_of(const GraphDev &g, int32_t u) {
+  return u < g.n_heavy ? g.nchunk[u] : 0;
+}
 
 constexpr int JF_NO_PRUNE = 4;  // == GST_F_NO_PRUNE
 constexpr int JF_NO_TREE = 2;   // == GST_F_NO_TREE
@@ -859,7 +865,7 @@ __global__ __launch_bounds__(BLK) void k_farsplit(const JobConst *jcs, JobState 
 // already set.
 // dbase/dstride: target u's level row is dbase + u * dstride.
 __device__ __forceinline__ void relax_pair(int32_t *dbase, size_t dstride, unsigned long long *act_lo,
-                                           const int32_t *nchunk, const RelaxGroup &G, bool h0, int32_t u0,
+                                           const GraphDev &g, const RelaxGroup &G, bool h0, int32_t u0,
                                            int32_t w0, bool h1, int32_t u1, int32_t w1, bool &want0,
                                            bool &want1, int &n0, int &n1, int &n_atom) {
   int32_t *Td0 = dbase + (size_t)u0 * dstride, *Td1 = dbase + (size_t)u1 * dstride;
@@ -890,8 +896,8 @@ __device__ __forceinline__ void relax_pair(int32_t *dbase, size_t dstride, unsig
   const unsigned long long cur0 = imp0 ? act_lo[u0] : ~0ull, cur1 = imp1 ? act_lo[u1] : ~0ull;
   want0 = (imp0 & ~cur0) ? atomicOr(&act_lo[u0], imp0) == 0 : false;
   want1 = (imp1 & ~cur1) ? atomicOr(&act_lo[u1], imp1) == 0 : false;
-  n0 = want0 ? nchunk[u0] : 0;
-  n1 = want1 ? nchunk[u1] : 0;
+  n0 = want0 ? chunks_of(g, u0) : 0;
+  n1 = want1 ? chunks_of(g, u1) : 0;
 }
 
 // Thread per vertex (degree < split). The edge loop is uniform across the wavefront
@@ -953,6 +959,15 @@ __global__ __launch_bounds__(BLK) void k_grow_thread(const JobConst *jcs, JobSta
     int ei = 0;
     bool live = gm != 0;
     if (live) take_group(src, gm, G);
+    // the next arc pair is loaded one step ahead, so its latency overlaps the target
+    // loads of the current pair
+    int32_t nu0 = 0, nw0 = 0, nu1 = 0, nw1 = 0;
+    if (live) {
+      nu0 = adj[e0];
+      nw0 = wts[e0];
+      nu1 = deg > 1 ? adj[e0 + 1] : nu0;
+      nw1 = deg > 1 ? wts[e0 + 1] : nw0;
+    }
     for (;;) {
       if (!__any(live)) break;
       bool want0 = false, want1 = false;
@@ -960,17 +975,24 @@ __global__ __launch_bounds__(BLK) void k_grow_thread(const JobConst *jcs, JobSta
       int n0 = 0, n1 = 0;
       if (live) {
         const bool two = ei + 1 < deg;
-        u0 = adj[e0 + ei];
-        const int32_t w0 = wts[e0 + ei];
-        u1 = two ? adj[e0 + ei + 1] : u0;
-        const int32_t w1 = two ? wts[e0 + ei + 1] : w0;
-        relax_pair(dbase, dstride, act_lo, g.nchunk, G, true, u0, w0, two, u1, w1, want0, want1, n0, n1, k.atom);
+        u0 = nu0;
+        const int32_t w0 = nw0;
+        u1 = nu1;
+        const int32_t w1 = nw1;
+        RelaxGroup cur = G;
         ei += two ? 2 : 1;
         if (ei >= deg) {
           ei = 0;
           if (gm) take_group(src, gm, G);
           else live = false;
         }
+        if (live) {
+          nu0 = adj[e0 + ei];
+          nw0 = wts[e0 + ei];
+          nu1 = ei + 1 < deg ? adj[e0 + ei + 1] : nu0;
+          nw1 = ei + 1 < deg ? wts[e0 + ei + 1] : nw0;
+        }
+        relax_pair(dbase, dstride, act_lo, g, cur, true, u0, w0, two, u1, w1, want0, want1, n0, n1, k.atom);
       }
       wp_push(pb, jcs, jss, g, jb, lo, want0, u0, n0);
       wp_push(pb, jcs, jss, g, jb, lo, want1, u1, n1);
@@ -1038,7 +1060,7 @@ __global__ __launch_bounds__(BLK) void k_grow_wave(const JobConst *jcs, JobState
           const int32_t w0 = wts[e0];
           u1 = h1 ? adj[e1] : u0;
           const int32_t w1 = h1 ? wts[e1] : w0;
-          relax_pair(dbase, dstride, act_lo, g.nchunk, G, true, u0, w0, h1, u1, w1, want0, want1, n0, n1, k.atom);
+          relax_pair(dbase, dstride, act_lo, g, G, true, u0, w0, h1, u1, w1, want0, want1, n0, n1, k.atom);
         }
         wp_push(pb, jcs, jss, g, jb, lo, want0, u0, n0);
         wp_push(pb, jcs, jss, g, jb, lo, want1, u1, n1);
@@ -1452,7 +1474,7 @@ __global__ __launch_bounds__(BLK) void k_collect(const JobConst *jcs, JobState *
     const int li = c.hop ? (int)(r / nblk) : (s->parity ^ 1);
     const long long v = (c.hop ? r % nblk : r) * 64 + lane_id();
     const bool want = v < g.n && c.act[(size_t)li * g.n + v] != 0;
-    const int nch = want ? g.nchunk[v] : 0;
+    const int nch = want ? chunks_of(g, (int32_t)v) : 0;
     wp_push(pb, jcs, jss, g, jb, li, want, (int32_t)v, nch);
   }
   wp_flush(pb, jcs, jss, g);

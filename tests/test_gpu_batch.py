@@ -47,6 +47,23 @@ def test_batch_matches_plain_and_oracle(gpu_engine, rmat, rmat_inst):
         _assert_same(rmat, rmat_inst, q, r, alone)
 
 
+def test_batch_after_tuning_change(rmat, rmat_inst):
+    """The batched rounds are captured with the degree split and chunk size of their
+    time; a tuning change between two batched calls must not run stale ones."""
+    eng = engine.Engine(rmat.graph, rmat.groups, device=0)
+    try:
+        qs = _mixed_queries(rmat, 12, 5)
+        for split, chunk in [(16, 64), (48, 128), (256, 4096)]:
+            eng.set_tuning(split, chunk)
+            print("tuning", split, chunk, flush=True)
+            res = eng.solve_batch(qs)
+            for q, r in zip(qs, res):
+                alone = eng.solve(q.gids, D=q.D)
+                _assert_same(rmat, rmat_inst, q, r, alone)
+    finally:
+        eng.close()
+
+
 def test_batch_small_budget_serialises(rmat, rmat_inst):
     # a budget that fits the largest arena of the set, but never all of them at once,
     # forces queuing; results unchanged
