@@ -157,3 +157,22 @@ def test_oracle_rejects_bad_input(rmat, rmat_inst):
     # duplicates are dropped like the reference's unordered_set
     q = list(rmat.queries[3][0])
     assert po.prunedpp(rmat_inst, q + q[:1])["tree_cost"] == po.prunedpp(rmat_inst, q)["tree_cost"]
+
+
+def test_edge_case_table_matches_oracle():
+    """The stated answers of the hand-built GPU edge cases (tests/test_gpu_edge.py) are the
+    oracle's: PrunedDP++ (D < 0) or D-PrunedDP++ on the same graph and groups."""
+    import test_gpu_edge as te
+    g = ds.from_edges(8, te._U, te._V, te._W)
+    offs = np.zeros(len(te._MEMBERS) + 1, np.int64)
+    offs[1:] = np.cumsum([len(m) for m in te._MEMBERS])
+    gr = ds.Groups(offs, np.array([x for m in te._MEMBERS for x in m], np.int32))
+    inst = po.Instance(g, gr)
+    for gids, D, cost, edges in te.CASES:
+        q = np.array(gids, np.int32)
+        o = po.prunedpp(inst, q) if D < 0 else po.dprunedpp(inst, q, D)
+        if cost is None:
+            assert o["status"] != 0, (gids, D, o)
+        else:
+            assert o["status"] == 0 and o["tree_cost"] == cost, (gids, D, o)
+            assert sum(e[2] for e in edges) == cost
