@@ -189,7 +189,7 @@ def main():
         tuned = {"degree_split": args.tuning[0], "chunk_edges": args.tuning[1], "near_far_delta": 0}
     elif not args.no_tune:
         sample = queries[-3:]
-        tuned, _ = autotune.tune(eng, sample, grid={"degree_split": (128, 256, 512), "chunk_edges": (4096, 16384),
+        tuned, _ = autotune.tune(eng, sample, grid={"degree_split": (32, 64, 128), "chunk_edges": (512, 1024, 4096),
                                                    "near_far_delta": (0,)}, budget_s=30.0)
         print(f"# autotuned {tuned} ({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
     flags = engine.F_STATS if args.stats else 0

@@ -14,8 +14,8 @@ import time
 import numpy as np
 
 DEFAULT_GRID = {
-    "degree_split": (64, 128, 256, 512),
-    "chunk_edges": (1024, 4096, 16384),
+    "degree_split": (16, 32, 64, 128, 256),
+    "chunk_edges": (512, 1024, 4096),
     "near_far_delta": (0, 64),
 }
 

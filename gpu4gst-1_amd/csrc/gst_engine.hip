@@ -90,7 +90,7 @@ struct gst_ctx {
   int64_t graph_bytes = 0;
   long long bcap = 0;  // wavefront items one worklist can hold
   gst_opts opts{};
-  int32_t split = 256, chunk = 4096, delta = 0;
+  int32_t split = 32, chunk = 1024, delta = 0;
   // single-query path: one slot per (K, D) shape, least recently used evicted beyond
   // plain_cap bytes (a mixed stream of shapes would otherwise reallocate its arena,
   // up to V * 2^k * (D+1) * 4 bytes, on almost every query)
@@ -497,8 +497,8 @@ int gst_open(const gst_csr *csr, const gst_groups *groups, const gst_opts *opts,
   c->n = n;
   c->m = m;
   c->n_groups = groups->n_groups;
-  c->split = c->opts.degree_split > 0 ? c->opts.degree_split : 256;
-  c->chunk = c->opts.chunk_edges > 0 ? ((c->opts.chunk_edges + 63) / 64) * 64 : 4096;
+  c->split = c->opts.degree_split > 0 ? c->opts.degree_split : 32;
+  c->chunk = c->opts.chunk_edges > 0 ? ((c->opts.chunk_edges + 63) / 64) * 64 : 1024;
   c->delta = c->opts.near_far_delta > 0 ? c->opts.near_far_delta : 0;
   c->max_batch = c->opts.max_batch > 0 ? c->opts.max_batch : 64;
   DeviceGraph dg = renumber(csr, groups, !(c->opts.flags & GST_O_KEEP_ORDER));

@@ -82,8 +82,8 @@ typedef struct {
 typedef struct {
   int32_t device;          /* HIP device ordinal */
   int32_t degree_split;    /* vertices with degree >= this are grown by a wavefront per
-                              edge chunk, the rest by a thread per vertex (0 = 256) */
-  int32_t chunk_edges;     /* edges per wavefront work item (0 = 4096) */
+                              edge chunk, the rest by a thread per vertex (0 = 32) */
+  int32_t chunk_edges;     /* edges per wavefront work item (0 = 1024) */
   int32_t max_batch;       /* job slots of the batched/served forms (0 = 64) */
   int64_t dp_budget_bytes; /* DP memory budget of the batched/served forms
                               (0 = 75% of free device memory at gst_open) */
