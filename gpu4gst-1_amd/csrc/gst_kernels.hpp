@@ -1021,26 +1021,46 @@ __global__ __launch_bounds__(BLK) void k_grow_wave(const JobConst *jcs, JobState
   const int32_t *__restrict__ wts = g.w;
   const long long waves_per_grid = (long long)gridDim.x * (BLK / WAVE);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: scalar registers
-  for (long long it = (long long)blockIdx.x * (BLK / WAVE) + wave; it < total; it += waves_per_grid) {
-    const int jb = find_job(soff, nj, it);
+  // The wave's items are it0, it0 + S, it0 + 2S, ... (S = waves in the grid). Each lane
+  // fetches the header of one of the next 64 (vertex, chunk, active mask, arc range, job),
+  // so the header loads of 64 items share one memory latency; the wave then walks them.
+  for (long long base = (long long)blockIdx.x * (BLK / WAVE) + wave; base < total; base += WAVE * waves_per_grid) {
+    const long long my = base + (long long)lane * waves_per_grid;
+    int hjb = 0, hv = 0, heb = 0, hee = 0, hch = 0;
+    unsigned long long hgm = 0;
+    if (my < total) {
+      hjb = find_job(soff, nj, my);
+      const JobConst &c = jcs[hjb];
+      const int li = cur_list(c, &jss[hjb]);
+      const int32_t *item = c.blst + ((size_t)li * c.bcap + (my - soff[hjb])) * 2;
+      hv = item[0];
+      hch = item[1];
+      hgm = c.act[(size_t)li * g.n + hv];  // already reduced by k_select
+      heb = g.beg[hv] + hch * g.chunk;
+      hee = min(g.beg[hv + 1], heb + g.chunk);
+    }
+    const long long left = (total - base + waves_per_grid - 1) / waves_per_grid;
+    const int nitems = (int)(left < WAVE ? left : WAVE);
+    for (int l = 0; l < nitems; l++) {
+    const int jb = __builtin_amdgcn_readlane(hjb, l);
     if (jb != cj) { flush_lane(jcs, jss, cj, k); k.zero(); cj = jb; }
     const JobConst &c = jcs[jb];
     JobState *s = &jss[jb];
-    const int li = cur_list(c, s), lo = nxt_list(c, s);
+    const int lo = nxt_list(c, s);
     const int L = s->level;
     const int d = c.hop ? s->depth : 0;
     const size_t dstride = (size_t)c.H * c.nsub[L];
     int32_t *const dbase = lrow(c, L, 0, c.hop ? d + 1 : 0);
     unsigned long long *const act_lo = c.act + (size_t)lo * g.n;
-    const int32_t *item = c.blst + ((size_t)li * c.bcap + (it - soff[jb])) * 2;
-    const int32_t v = item[0], ch = item[1];
-    unsigned long long gm = c.act[(size_t)li * g.n + v];  // already reduced by k_select
+    const int32_t v = __builtin_amdgcn_readlane(hv, l), ch = __builtin_amdgcn_readlane(hch, l);
+    unsigned long long gm = ((unsigned long long)(uint32_t)__builtin_amdgcn_readlane((int)(hgm >> 32), l) << 32) |
+                            (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)hgm, l);
     int32_t *src = lrow(c, L, v, d);
     if (lane == 0 && ch == 0) k.verts++;
     if (!gm) continue;
     const int nb = popc((uint32_t)gm) + popc((uint32_t)(gm >> 32));
-    const int32_t eb = g.beg[v] + ch * g.chunk;
-    const int32_t ee = min(g.beg[v + 1], eb + g.chunk);
+    const int32_t eb = __builtin_amdgcn_readlane(heb, l);
+    const int32_t ee = __builtin_amdgcn_readlane(hee, l);
     if (lane == 0) {
       k.edges += ee - eb;
       k.relax += (ee - eb) * nb;
@@ -1066,6 +1086,7 @@ __global__ __launch_bounds__(BLK) void k_grow_wave(const JobConst *jcs, JobState
         wp_push(pb, jcs, jss, g, jb, lo, want1, u1, n1);
         k.push += (int)want0 + (int)want1;
       }
+    }
     }
   }
   wp_flush(pb, jcs, jss, g);
