@@ -174,24 +174,27 @@ def main():
     print(f"# workload {args.workload}: n={g.n} edges={g.n_edges} groups={wl.groups.n_groups} "
           f"(generated in {time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
     need = args.batch * (args.steps + args.warmup)
+    n_tune = 32  # queries after the batches, used only by the autotuner
     rng = np.random.default_rng(datasets.SEED_BASE + 77)
     pool = wl.queries[args.k]
     extra = [rng.choice(wl.groups.n_groups, args.k, replace=False).astype(np.int32)
-             for _ in range(max(0, need - len(pool)))]
+             for _ in range(max(0, need + n_tune - len(pool)))]
     queries = [np.asarray(q, np.int32) for q in pool] + extra
+    tune_q = queries[need:need + n_tune]
     queries = queries[:need]
 
     eng = engine.Engine(g, wl.groups, max_batch=args.batch)
     print(f"# opened engine ({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
-    tuned = None
+    tuned = tuned_b = None
     if args.tuning:
         eng.set_tuning(args.tuning[0], args.tuning[1], 0)
-        tuned = {"degree_split": args.tuning[0], "chunk_edges": args.tuning[1], "near_far_delta": 0}
+        tuned = tuned_b = {"degree_split": args.tuning[0], "chunk_edges": args.tuning[1], "near_far_delta": 0}
     elif not args.no_tune:
-        sample = queries[-3:]
-        tuned, _ = autotune.tune(eng, sample, grid={"degree_split": (32, 64, 128), "chunk_edges": (512, 1024, 4096),
-                                                   "near_far_delta": (0,)}, budget_s=30.0)
-        print(f"# autotuned {tuned} ({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
+        # tuned on queries outside every batch
+        tuned, _ = autotune.tune(eng, tune_q[:3], grid={"degree_split": (32, 64, 128),
+                                                         "chunk_edges": (512, 1024, 4096),
+                                                         "near_far_delta": (0,)}, budget_s=30.0)
+        print(f"# autotuned (plain) {tuned} ({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
     flags = engine.F_STATS if args.stats else 0
 
     # ---- plain path: one query at a time (also gives the grow-kernel roofline), on the
@@ -217,7 +220,12 @@ def main():
         return
     assert args.D < 0, "the batched benchmark runs the non-hop workload; --D is for --profile-only"
 
-    # ---- batched path: warmup then timed steps
+    # ---- batched path: its own tuning, warmup, then timed steps
+    if not args.tuning and not args.no_tune:
+        tuned_b, _ = autotune.tune(eng, tune_q[:min(n_tune, args.batch)],
+                                   grid={"degree_split": (64, 128, 256, 512), "chunk_edges": (1024, 4096, 16384),
+                                         "near_far_delta": (0,)}, budget_s=60.0, batched=True)
+        print(f"# autotuned (batched) {tuned_b} ({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
     batches = [queries[i * args.batch:(i + 1) * args.batch] for i in range(args.warmup + args.steps)]
     for b in batches[:args.warmup]:
         eng.solve_batch(b)
@@ -284,7 +292,8 @@ def main():
         "data": "synthetic (seeded Chung-Lu graph, Zipf groups, uniform queries)",
         "config": {"workload": f"{args.workload}: Chung-Lu beta=2.0, n={g.n}, undirected edges={g.n_edges}, "
                                f"{wl.groups.n_groups} groups, k={args.k}",
-                   "batch": args.batch, "plain_queries": len(plain), "tuning": tuned},
+                   "batch": args.batch, "plain_queries": len(plain), "tuning": tuned,
+                   "tuning_batched": tuned_b},
         "p50_ms": round(float(np.median(blat)), 3),
         "p99_ms": round(float(np.percentile(blat, 99)), 3),
         "plain": {"qps": round(plain_qps, 3), "p50_ms": round(float(np.median(lat)), 3),

@@ -20,10 +20,15 @@ DEFAULT_GRID = {
 }
 
 
-def tune(eng, queries, D=-1, grid=None, repeats=1, budget_s=60.0, log=None):
-    """Time every grid point on `queries` (plain path) and apply the fastest.
+def tune(eng, queries, D=-1, grid=None, repeats=1, budget_s=60.0, log=None, batched=False):
+    """Time every grid point on `queries` and apply the fastest: one query at a time on
+    the plain path, or, with batched=True, all of them as one batch (the batched form
+    favours other settings: with many jobs in flight, lanes idle behind a long thread
+    item are covered by other waves, while per-item overheads are not).
 
     Returns (best_config dict, table of (config, seconds per query))."""
+    from . import engine
+
     grid = grid or DEFAULT_GRID
     keys = list(grid)
     table = []
@@ -33,11 +38,18 @@ def tune(eng, queries, D=-1, grid=None, repeats=1, budget_s=60.0, log=None):
     for vals in itertools.product(*(grid[k] for k in keys)):
         cfg = dict(zip(keys, vals))
         eng.set_tuning(cfg["degree_split"], cfg["chunk_edges"], cfg["near_far_delta"])
-        eng.solve(queries[0], D=D)  # warm (allocations, code paths)
-        t = time.perf_counter()
-        costs = []
-        for _ in range(repeats):
-            costs = [eng.solve(q, D=D)["cost"] for q in queries]
+        if batched:
+            batch = [engine.Query(np.asarray(q, np.int32), D) for q in queries]
+            eng.solve_batch(batch[:2])  # warm (graph capture, slot pool)
+            t = time.perf_counter()
+            for _ in range(repeats):
+                costs = [r["cost"] for r in eng.solve_batch(batch)]
+        else:
+            eng.solve(queries[0], D=D)  # warm (allocations, code paths)
+            t = time.perf_counter()
+            costs = []
+            for _ in range(repeats):
+                costs = [eng.solve(q, D=D)["cost"] for q in queries]
         dt = (time.perf_counter() - t) / (repeats * len(queries))
         if ref is None:
             ref = costs
