@@ -112,3 +112,18 @@ def test_rmat_hop_pruned_matches_unpruned(gpu_engine, rmat, D):
         if a["status"] == engine.GST_OK:
             ok, why = engine.validate_tree(rmat.graph, rmat.groups, q, a["edges"], a["cost"], D)
             assert ok, (q, D, why)
+
+
+@pytest.mark.parametrize("D", [8, 15])
+def test_rmat_hop_large_D(gpu_engine, rmat, rmat_inst, D):
+    """Diameter bounds up to GST_MAX_D (16 heights per state: the largest hop tiles)."""
+    rng = np.random.default_rng(900 + D)
+    for i in range(3):
+        k = int(rng.integers(3, 5))
+        q = rng.choice(rmat.groups.n_groups, k, replace=False).astype(np.int32)
+        print("large D", D, "query", i, q, flush=True)
+        r = gpu_engine.solve(q, D=D)
+        o = pyoracle.dprunedpp(rmat_inst, q, D)
+        assert r["status"] == engine.GST_OK and r["cost"] == o["tree_cost"] == o["dp_cost"], (q, D, r["cost"], o)
+        ok, why = engine.validate_tree(rmat.graph, rmat.groups, q, r["edges"], r["cost"], D)
+        assert ok, (q, D, why)
