@@ -900,14 +900,29 @@ __device__ __forceinline__ void relax_pair(int32_t *dbase, size_t dstride, unsig
   n1 = want1 ? chunks_of(g, u1) : 0;
 }
 
-// Thread per vertex (degree < split). The edge loop is uniform across the wavefront
-// (lanes run (group, edge-pair) steps in lock step), so worklist pushes are aggregated.
-// Job fields are copied to registers: the arena atomics could otherwise alias them and
-// force a reload on every step.
+// Thread items (degree < split), edge-balanced. A wavefront takes 64 consecutive items and
+// spreads their arcs over its lanes: a wave-level prefix sum of the item degrees gives
+// each item's first arc slot, and a lane finds the item of its slot by binary search in
+// LDS, so no lane idles behind a longer item (degrees below the split vary several-fold).
+// Each item's job fields and its group of up to RG active source states are cached in
+// LDS; items with more active states take further passes. Every LDS hand-off stays inside
+// one wavefront.
+struct ThreadSeg {
+  int32_t off[WAVE + 1];  // first arc slot of each item; off[WAVE] = arcs of this pass
+  int32_t e0[WAVE];       // first CSR arc of the item
+  int32_t jb[WAVE], lo[WAVE];
+  int32_t gn[WAVE];       // states in this pass's group
+  int32_t gp[WAVE][RG], gc[WAVE][RG];
+  int32_t *dbase[WAVE];
+  unsigned long long *act[WAVE];
+  long long dstride[WAVE];
+};
+
 __global__ __launch_bounds__(BLK) void k_grow_thread(const JobConst *jcs, JobState *jss, const Plan *pl, GraphDev g) {
   __shared__ long long soff[MAXJOBS + 1];
   __shared__ int32_t pb_small[(BLK / WAVE) * PB_SMALL];
   __shared__ int32_t pb_big[(BLK / WAVE) * 2 * PB_BIG];
+  __shared__ ThreadSeg segs[BLK / WAVE];
   WavePush pb;
   wp_init(pb, pb_small, pb_big);
   const int nj = load_plan(pl, WK_THR, soff);
@@ -915,36 +930,28 @@ __global__ __launch_bounds__(BLK) void k_grow_thread(const JobConst *jcs, JobSta
   GrowCounters k;
   k.zero();
   int cj = -1;
+  const int lane = lane_id();
+  ThreadSeg &S = segs[threadIdx.x >> 6];
   const long long stride = (long long)gridDim.x * BLK;
   const int32_t *__restrict__ adj = g.adj;
   const int32_t *__restrict__ wts = g.w;
-  const long long base0 = (long long)blockIdx.x * BLK + (threadIdx.x & ~63);
-  for (long long wbase = base0; wbase < total; wbase += stride) {
-    const long long it = wbase + lane_id();
-    const bool has = it < total;
-    int jb = -1, lo = 0, L = 0, deg = 0;
-    int32_t v = 0, e0 = 0;
-    int32_t *src = nullptr, *dbase = nullptr;
-    unsigned long long *act_lo = nullptr;
-    size_t dstride = 0;
+  for (long long wbase = (long long)blockIdx.x * BLK + (threadIdx.x & ~63); wbase < total; wbase += stride) {
+    const long long it = wbase + lane;
+    int deg = 0;
+    const int32_t *src = nullptr;
     unsigned long long gm = 0;
-    if (has) {
-      jb = find_job(soff, nj, it);
+    if (it < total) {
+      const int jb = find_job(soff, nj, it);
       if (jb != cj) { flush_lane(jcs, jss, cj, k); k.zero(); cj = jb; }
       const JobConst &c = jcs[jb];
       JobState *s = &jss[jb];
-      const int li = cur_list(c, s);
-      lo = nxt_list(c, s);
-      L = s->level;
+      const int li = cur_list(c, s), lo = nxt_list(c, s), L = s->level;
       const int d = c.hop ? s->depth : 0;
-      dstride = (size_t)c.H * c.nsub[L];
-      dbase = lrow(c, L, 0, c.hop ? d + 1 : 0);
-      act_lo = c.act + (size_t)lo * g.n;
-      v = c.lst[(size_t)li * g.n + (it - soff[jb])];
+      const int32_t v = c.lst[(size_t)li * g.n + (it - soff[jb])];
       gm = c.act[(size_t)li * g.n + v];  // already reduced by k_select
       k.verts++;
       src = lrow(c, L, v, d);
-      e0 = g.beg[v];
+      const int32_t e0 = g.beg[v];
       deg = g.beg[v + 1] - e0;
       if (gm && deg) {
         const int nb = popc((uint32_t)gm) + popc((uint32_t)(gm >> 32));
@@ -953,50 +960,60 @@ __global__ __launch_bounds__(BLK) void k_grow_thread(const JobConst *jcs, JobSta
       } else {
         gm = 0;
       }
+      S.e0[lane] = e0;
+      S.jb[lane] = jb;
+      S.lo[lane] = lo;
+      S.dbase[lane] = lrow(c, L, 0, c.hop ? d + 1 : 0);
+      S.dstride[lane] = (long long)c.H * c.nsub[L];
+      S.act[lane] = c.act + (size_t)lo * g.n;
     }
-    RelaxGroup G;
-    G.n = 0;
-    int ei = 0;
-    bool live = gm != 0;
-    if (live) take_group(src, gm, G);
-    // the next arc pair is loaded one step ahead, so its latency overlaps the target
-    // loads of the current pair
-    int32_t nu0 = 0, nw0 = 0, nu1 = 0, nw1 = 0;
-    if (live) {
-      nu0 = adj[e0];
-      nw0 = wts[e0];
-      nu1 = deg > 1 ? adj[e0 + 1] : nu0;
-      nw1 = deg > 1 ? wts[e0 + 1] : nw0;
-    }
-    for (;;) {
-      if (!__any(live)) break;
-      bool want0 = false, want1 = false;
-      int32_t u0 = 0, u1 = 0;
-      int n0 = 0, n1 = 0;
-      if (live) {
-        const bool two = ei + 1 < deg;
-        u0 = nu0;
-        const int32_t w0 = nw0;
-        u1 = nu1;
-        const int32_t w1 = nw1;
-        RelaxGroup cur = G;
-        ei += two ? 2 : 1;
-        if (ei >= deg) {
-          ei = 0;
-          if (gm) take_group(src, gm, G);
-          else live = false;
-        }
-        if (live) {
-          nu0 = adj[e0 + ei];
-          nw0 = wts[e0 + ei];
-          nu1 = ei + 1 < deg ? adj[e0 + ei + 1] : nu0;
-          nw1 = ei + 1 < deg ? wts[e0 + ei + 1] : nw0;
-        }
-        relax_pair(dbase, dstride, act_lo, g, cur, true, u0, w0, two, u1, w1, want0, want1, n0, n1, k.atom);
+    for (;;) {  // one pass per group of up to RG active states of every item
+      RelaxGroup G;
+      G.n = 0;
+      if (gm) take_group(src, gm, G);
+      const int cnt = G.n ? deg : 0;
+      if (!__any(cnt > 0)) break;
+      S.gn[lane] = G.n;
+#pragma unroll
+      for (int i = 0; i < RG; i++) { S.gp[lane][i] = G.p[i]; S.gc[lane][i] = G.c[i]; }
+      int x = cnt;
+#pragma unroll
+      for (int o = 1; o < WAVE; o <<= 1) {
+        const int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
       }
-      wp_push(pb, jcs, jss, g, jb, lo, want0, u0, n0);
-      wp_push(pb, jcs, jss, g, jb, lo, want1, u1, n1);
-      k.push += (int)want0 + (int)want1;
+      S.off[lane] = x - cnt;
+      const int T = __shfl(x, WAVE - 1, 64);
+      if (lane == WAVE - 1) S.off[WAVE] = x;
+      __builtin_amdgcn_wave_barrier();
+      for (int a0 = 0; a0 < T; a0 += WAVE) {  // uniform trip count
+        const int a = a0 + lane;
+        bool want = false, want1 = false;
+        int32_t u = 0;
+        int nch = 0, n1 = 0, ljb = -1, llo = 0;
+        if (a < T) {
+          int lo_ = 0, hi = WAVE - 1;  // the last item whose first slot is <= a
+          while (lo_ < hi) {
+            const int mid = (lo_ + hi + 1) >> 1;
+            if (S.off[mid] <= a) lo_ = mid; else hi = mid - 1;
+          }
+          const int j = lo_;
+          const int32_t e = S.e0[j] + (a - S.off[j]);
+          u = adj[e];
+          const int32_t w = wts[e];
+          RelaxGroup Gj;
+          Gj.n = S.gn[j];
+#pragma unroll
+          for (int i = 0; i < RG; i++) { Gj.p[i] = S.gp[j][i]; Gj.c[i] = S.gc[j][i]; }
+          relax_pair(S.dbase[j], (size_t)S.dstride[j], S.act[j], g, Gj, true, u, w, false, u, w, want, want1,
+                     nch, n1, k.atom);
+          ljb = S.jb[j];
+          llo = S.lo[j];
+        }
+        wp_push(pb, jcs, jss, g, ljb, llo, want, u, nch);
+        k.push += (int)want;
+      }
+      __builtin_amdgcn_wave_barrier();  // the next pass rewrites the segment
     }
   }
   wp_flush(pb, jcs, jss, g);
