@@ -549,7 +549,12 @@ int gst_open(const gst_csr *csr, const gst_groups *groups, const gst_opts *opts,
 }
 
 int gst_set_tuning(gst_ctx *c, int32_t degree_split, int32_t chunk_edges, int32_t near_far_delta) {
-  if (!c || degree_split < 1 || chunk_edges < 64 || near_far_delta < 0) { set_err("bad tuning"); return GST_BAD_INPUT; }
+  // a wavefront of thread items sums 64 degrees below the split in 32 bits
+  if (!c || degree_split < 1 || degree_split > (1 << 24) || chunk_edges < 64 || chunk_edges > (1 << 24) ||
+      near_far_delta < 0) {
+    set_err("bad tuning (need 1 <= degree_split <= 2^24, 64 <= chunk_edges <= 2^24, near_far_delta >= 0)");
+    return GST_BAD_INPUT;
+  }
   c->delta = near_far_delta;
   chunk_edges = ((chunk_edges + 63) / 64) * 64;
   long long bcap = 0;

@@ -14,8 +14,8 @@
 // of that size contiguously (ascending p, so bit j of a level mask is offset j):
 //   T[off[L] + ((v*H + d) * C(k,L)) + rank(p)],  H = D+1 (1 without a diameter bound).
 // A grow round of level L touches only that block (k=5: 20-40 B per vertex row), so a
-// relaxation reads one or two 64-B sectors and the level's working set is a fraction of
-// the arena.
+// relaxation reads one 128-B line of the target and the level's working set is a
+// fraction of the arena.
 //
 // Non-hop schedule (level = popcount of p):
 //   level 1 : k-source SSSP = the singleton states (grow rounds)          [SSSP, a4]
@@ -34,9 +34,12 @@
 //     CPUNONHOP.h:436-496).
 //   Level-1 growth uses only the UB/2 rule, so every singleton value <= UB_final/2 is an
 //   exact distance and dl_i(v) = min(T(v,{i}), UB_final/2 + 1) is a valid lower bound.
-// Hop schedule: no pruning (the reference's D-PrunedDP++ and D-TrimCDP prune nothing);
-//   per level the merge kernel fills all heights, then heights 0..D-1 are grown in order
-//   (a grow raises the height by exactly one, so each height is final before it is grown).
+// Hop schedule: per level the merge kernel fills all heights, then heights 0..D-1 are
+//   grown in order (a grow raises the height by exactly one, so each height is final
+//   before it is grown). A stored hop state holds a tree of height at most d. Pruning
+//   (the reference's D-PrunedDP++ and D-TrimCDP prune nothing): cost + lb > UB with lb
+//   from D-d hop group distances and UB from stars and complement merges
+//   (grow_select_hop); no UB/2 rule, since re-rooting changes heights.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -424,7 +427,7 @@ __global__ __launch_bounds__(BLK) void k_seed(const JobConst *jcs, JobState *jss
 
 // --------------------------------------------------------------------------- grow
 // Evaluate the active states of v (bits of a) at the current level: update UB, and
-// return the subset of bits that may grow. Non-hop only.
+// return the subset of bits that may grow. Non-hop jobs (grow_select_hop for hop jobs).
 // Non-hop: v's active states of level L are bits of a (bit j = offset j of v's level row).
 __device__ __forceinline__ unsigned long long grow_select(const JobConst &c, JobState *s,
                                                           int32_t v, unsigned long long a,

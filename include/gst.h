@@ -144,8 +144,9 @@ int gst_submit(gst_ctx *ctx, const gst_query *queries, int32_t n, uint64_t *tick
 int gst_poll(gst_ctx *ctx, uint64_t ticket, gst_result *out);
 int gst_wait(gst_ctx *ctx, uint64_t ticket, gst_result *out);
 int gst_info_get(gst_ctx *ctx, gst_info *info);
-/* Tunables of the grow phase (degree split, chunk size, near-far step, 0 = off); used by
- * the autotuner. Call while no batched or served query is in flight. */
+/* Tunables of the grow phase (degree split in [1, 2^24], chunk size in [64, 2^24] arcs,
+ * near-far step, 0 = off); used by the autotuner. Call while no batched or served query
+ * is in flight. */
 int gst_set_tuning(gst_ctx *ctx, int32_t degree_split, int32_t chunk_edges, int32_t near_far_delta);
 void gst_result_free(gst_result *r);
 void gst_close(gst_ctx *ctx);

@@ -79,3 +79,13 @@ def test_tiny_graph_rejects_bad_queries(tiny, gids, D):
     with pytest.raises(engine.GstError) as e:
         eng.solve(np.array(gids, np.int32), D=D)
     assert e.value.code == engine.GST_BAD_INPUT
+
+
+@pytest.mark.parametrize("split, chunk", [(0, 1024), (1 << 25, 1024), (32, 32), (32, 1 << 25)])
+def test_tuning_rejects_out_of_range(tiny, split, chunk):
+    g, gr, eng = tiny
+    with pytest.raises(engine.GstError) as e:
+        eng.set_tuning(split, chunk)
+    assert e.value.code == engine.GST_BAD_INPUT
+    eng.set_tuning(32, 1024)  # still usable
+    assert eng.solve(np.array([0, 1], np.int32))["cost"] == 6
