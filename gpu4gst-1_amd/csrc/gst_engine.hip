@@ -459,8 +459,10 @@ int gst_open(const gst_csr *csr, const gst_groups *groups, const gst_opts *opts,
   *out = nullptr;
   const int32_t n = csr->n;
   const int64_t m = csr->m;
-  if (n < 1 || m < 0 || m >= (1ll << 31) || !csr->beg_pos || (m && (!csr->adj || !csr->w))) {
-    set_err("bad CSR sizes (need n >= 1, m < 2^31)");
+  // arc offsets are int32 on the device, and a kernel may form an offset up to one chunk
+  // (at most 2^24 arcs) or one strip past the last arc: m stays 2^25 below 2^31
+  if (n < 1 || m < 0 || m > GST_MAX_ARCS || !csr->beg_pos || (m && (!csr->adj || !csr->w))) {
+    set_err("bad CSR sizes (need n >= 1, m <= 2^31 - 2^25)");
     return GST_BAD_INPUT;
   }
   if (csr->beg_pos[0] != 0 || csr->beg_pos[n] != m) { set_err("beg_pos must run from 0 to m"); return GST_BAD_INPUT; }

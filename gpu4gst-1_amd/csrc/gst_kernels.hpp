@@ -862,45 +862,47 @@ __global__ __launch_bounds__(BLK) void k_farsplit(const JobConst *jcs, JobState 
 }
 
 // --------------------------------------------------------------------------- grow
-// Relax up to two edges of one lane for the subsets of G. The target states of both
-// edges are loaded in one batch; the arena atomicMin is fire-and-forget; the mask
-// atomicOr (whose result decides who pushes the target) is skipped when the bits are
-// already set.
-// dbase/dstride: target u's level row is dbase + u * dstride.
-__device__ __forceinline__ void relax_pair(int32_t *dbase, size_t dstride, unsigned long long *act_lo,
-                                           const GraphDev &g, const RelaxGroup &G, bool h0, int32_t u0,
-                                           int32_t w0, bool h1, int32_t u1, int32_t w1, bool &want0,
-                                           bool &want1, int &n0, int &n1, int &n_atom) {
-  int32_t *Td0 = dbase + (size_t)u0 * dstride, *Td1 = dbase + (size_t)u1 * dstride;
-  int t0[RG], t1[RG];
+// Relax up to NE edges of one lane for the subsets of G. The target states of all NE
+// edges are loaded in one batch (NE * G.n loads in flight per lane); the arena atomicMin
+// is fire-and-forget; the mask atomicOr (whose result decides who pushes the target) is
+// skipped when the bits are already set.
+// dbase/dstride: target u's level row is dbase + u * dstride. Edges with h[j] false are
+// skipped (want[j] = false, nch[j] = 0).
+template <int NE>
+__device__ __forceinline__ void relax_edges(int32_t *dbase, size_t dstride, unsigned long long *act_lo,
+                                            const GraphDev &g, const RelaxGroup &G, const bool (&h)[NE],
+                                            const int32_t (&u)[NE], const int32_t (&w)[NE], bool (&want)[NE],
+                                            int (&nch)[NE], int &n_atom) {
+  int32_t *Td[NE];
+  int t[NE][RG];
 #pragma unroll
-  for (int i = 0; i < RG; i++) {
-    t0[i] = (h0 && i < G.n) ? Td0[G.p[i]] : 0;
-    t1[i] = (h1 && i < G.n) ? Td1[G.p[i]] : 0;
+  for (int j = 0; j < NE; j++) {
+    Td[j] = dbase + (size_t)u[j] * dstride;
+#pragma unroll
+    for (int i = 0; i < RG; i++) t[j][i] = (h[j] && i < G.n) ? Td[j][G.p[i]] : 0;
   }
-  unsigned long long imp0 = 0, imp1 = 0;
+  unsigned long long imp[NE];
 #pragma unroll
-  for (int i = 0; i < RG; i++) {
-    if (i < G.n) {
-      if (h0) {
-        int nc = G.c[i] + w0;
+  for (int j = 0; j < NE; j++) {
+    imp[j] = 0;
+#pragma unroll
+    for (int i = 0; i < RG; i++) {
+      if (h[j] && i < G.n) {
+        int nc = G.c[i] + w[j];
         nc = nc < INF ? nc : INF;
-        if (nc < t0[i]) { atomicMin(&Td0[G.p[i]], nc); n_atom++; imp0 |= 1ull << G.p[i]; }
-      }
-      if (h1) {
-        int nd = G.c[i] + w1;
-        nd = nd < INF ? nd : INF;
-        if (nd < t1[i]) { atomicMin(&Td1[G.p[i]], nd); n_atom++; imp1 |= 1ull << G.p[i]; }
+        if (nc < t[j][i]) { atomicMin(&Td[j][G.p[i]], nc); n_atom++; imp[j] |= 1ull << G.p[i]; }
       }
     }
   }
   // masks and chunk counts are read only for targets that improved: every extra
-  // per-edge load would fetch another random 64-B sector
-  const unsigned long long cur0 = imp0 ? act_lo[u0] : ~0ull, cur1 = imp1 ? act_lo[u1] : ~0ull;
-  want0 = (imp0 & ~cur0) ? atomicOr(&act_lo[u0], imp0) == 0 : false;
-  want1 = (imp1 & ~cur1) ? atomicOr(&act_lo[u1], imp1) == 0 : false;
-  n0 = want0 ? chunks_of(g, u0) : 0;
-  n1 = want1 ? chunks_of(g, u1) : 0;
+  // per-edge load would fetch another random line
+  unsigned long long cur[NE];
+#pragma unroll
+  for (int j = 0; j < NE; j++) cur[j] = imp[j] ? act_lo[u[j]] : ~0ull;
+#pragma unroll
+  for (int j = 0; j < NE; j++) want[j] = (imp[j] & ~cur[j]) ? atomicOr(&act_lo[u[j]], imp[j]) == 0 : false;
+#pragma unroll
+  for (int j = 0; j < NE; j++) nch[j] = want[j] ? chunks_of(g, u[j]) : 0;
 }
 
 // Thread items (degree < split), edge-balanced. A wavefront takes 64 consecutive items and
@@ -991,9 +993,9 @@ __global__ __launch_bounds__(BLK) void k_grow_thread(const JobConst *jcs, JobSta
       __builtin_amdgcn_wave_barrier();
       for (int a0 = 0; a0 < T; a0 += WAVE) {  // uniform trip count
         const int a = a0 + lane;
-        bool want = false, want1 = false;
-        int32_t u = 0;
-        int nch = 0, n1 = 0, ljb = -1, llo = 0;
+        bool want[1] = {false};
+        int32_t u[1] = {0};
+        int nch[1] = {0}, ljb = -1, llo = 0;
         if (a < T) {
           int lo_ = 0, hi = WAVE - 1;  // the last item whose first slot is <= a
           while (lo_ < hi) {
@@ -1002,19 +1004,19 @@ __global__ __launch_bounds__(BLK) void k_grow_thread(const JobConst *jcs, JobSta
           }
           const int j = lo_;
           const int32_t e = S.e0[j] + (a - S.off[j]);
-          u = adj[e];
-          const int32_t w = wts[e];
+          u[0] = adj[e];
+          const int32_t w[1] = {wts[e]};
+          const bool h[1] = {true};
           RelaxGroup Gj;
           Gj.n = S.gn[j];
 #pragma unroll
           for (int i = 0; i < RG; i++) { Gj.p[i] = S.gp[j][i]; Gj.c[i] = S.gc[j][i]; }
-          relax_pair(S.dbase[j], (size_t)S.dstride[j], S.act[j], g, Gj, true, u, w, false, u, w, want, want1,
-                     nch, n1, k.atom);
+          relax_edges<1>(S.dbase[j], (size_t)S.dstride[j], S.act[j], g, Gj, h, u, w, want, nch, k.atom);
           ljb = S.jb[j];
           llo = S.lo[j];
         }
-        wp_push(pb, jcs, jss, g, ljb, llo, want, u, nch);
-        k.push += (int)want;
+        wp_push(pb, jcs, jss, g, ljb, llo, want[0], u[0], nch[0]);
+        k.push += (int)want[0];
       }
       __builtin_amdgcn_wave_barrier();  // the next pass rewrites the segment
     }
@@ -1024,7 +1026,10 @@ __global__ __launch_bounds__(BLK) void k_grow_thread(const JobConst *jcs, JobSta
 }
 
 // Wavefront per (vertex, edge chunk) for vertices with degree >= split. Each lane
-// relaxes two edges per strip of 128.
+// relaxes GW_NE edges per strip of GW_NE * 64, with all their target loads in flight
+// together: most arcs of this kernel sit in chunks of 512-1024 edges, which then take
+// 2-4 dependent round-trip chains instead of 4-8.
+constexpr int GW_NE = 4;
 __global__ __launch_bounds__(BLK) void k_grow_wave(const JobConst *jcs, JobState *jss, const Plan *pl, GraphDev g) {
   __shared__ long long soff[MAXJOBS + 1];
   __shared__ int32_t pb_small[(BLK / WAVE) * PB_SMALL];
@@ -1089,22 +1094,23 @@ __global__ __launch_bounds__(BLK) void k_grow_wave(const JobConst *jcs, JobState
       RelaxGroup G;
       take_group(src, gm, G);
       uniform_group(G);
-      for (int32_t e = eb; e < ee; e += 2 * WAVE) {  // uniform trip count
-        const int32_t e0 = e + lane, e1 = e + WAVE + lane;
-        const bool h0 = e0 < ee, h1 = e1 < ee;
-        bool want0 = false, want1 = false;
-        int32_t u0 = 0, u1 = 0;
-        int n0 = 0, n1 = 0;
-        if (h0) {
-          u0 = adj[e0];
-          const int32_t w0 = wts[e0];
-          u1 = h1 ? adj[e1] : u0;
-          const int32_t w1 = h1 ? wts[e1] : w0;
-          relax_pair(dbase, dstride, act_lo, g, G, true, u0, w0, h1, u1, w1, want0, want1, n0, n1, k.atom);
+      for (int32_t e = eb; e < ee; e += GW_NE * WAVE) {  // uniform trip count
+        bool h[GW_NE], want[GW_NE];
+        int32_t u[GW_NE], w[GW_NE];
+        int nch[GW_NE];
+#pragma unroll
+        for (int j = 0; j < GW_NE; j++) {
+          const int32_t ej = e + j * WAVE + lane;
+          h[j] = ej < ee;
+          u[j] = h[j] ? adj[ej] : 0;
+          w[j] = h[j] ? wts[ej] : 0;
         }
-        wp_push(pb, jcs, jss, g, jb, lo, want0, u0, n0);
-        wp_push(pb, jcs, jss, g, jb, lo, want1, u1, n1);
-        k.push += (int)want0 + (int)want1;
+        relax_edges<GW_NE>(dbase, dstride, act_lo, g, G, h, u, w, want, nch, k.atom);
+#pragma unroll
+        for (int j = 0; j < GW_NE; j++) {
+          wp_push(pb, jcs, jss, g, jb, lo, want[j], u[j], nch[j]);
+          k.push += (int)want[j];
+        }
       }
     }
     }

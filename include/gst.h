@@ -55,6 +55,8 @@ enum {
 
 #define GST_MAX_K 7
 #define GST_MAX_D 15
+/* Arc offsets are 32-bit on the device, with room for one chunk past the end. */
+#define GST_MAX_ARCS ((int64_t)2147483648 - 33554432)
 
 /* flags */
 #define GST_F_STATS 1    /* count finite states after the solve (extra arena scan) */
@@ -67,7 +69,7 @@ enum {
 
 typedef struct {
   int32_t n;              /* vertices */
-  int64_t m;              /* directed arcs = 2 x undirected edges; must be < 2^31 */
+  int64_t m;              /* directed arcs = 2 x undirected edges; at most GST_MAX_ARCS */
   const int64_t *beg_pos; /* [n+1] */
   const int32_t *adj;     /* [m], each adjacency list ascending */
   const int32_t *w;       /* [m], symmetric, in [1, 2^20] */
