@@ -131,6 +131,39 @@ def calibrate_copy(mib):
                                  "output_kib": g * 4 // 1024}}))
 
 
+def gather_ceiling():
+    """Rate of independent random 4-B reads on one GPU: torch.index_select of 2^25 uniform
+    random int32 elements from arrays of 64 MiB to 4 GiB, timed with HIP events on torch's
+    stream (10 launches after 3 warm-ups). A grow round's target-state reads have the same
+    pattern (one random 4-B read per arc over a level of the arena, 61-123 MB for the k=5
+    workload), without its dependent loads and atomics; this is the ceiling for them."""
+    import torch
+
+    torch.manual_seed(0)
+    n_idx = 1 << 25
+    rows = []
+    for mib in (64, 256, 1024, 4096):
+        n = mib * (1 << 20) // 4
+        x = torch.randint(0, 1000, (n,), dtype=torch.int32, device="cuda")
+        idx = torch.randint(0, n, (n_idx,), dtype=torch.int64, device="cuda")
+        for _ in range(3):
+            y = torch.index_select(x, 0, idx)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(10):
+            y = torch.index_select(x, 0, idx)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / 10
+        rows.append({"array_mib": mib, "reads": n_idx, "ms": round(ms, 4),
+                     "g_reads_per_s": round(n_idx / ms / 1e6, 2),
+                     "line_tb_per_s": round(n_idx * 128 / ms / 1e9, 2)})
+        del x, idx, y
+        torch.cuda.empty_cache()
+    print(json.dumps({"gather_ceiling": rows}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -152,11 +185,15 @@ def main():
     ap.add_argument("--rates", type=float, nargs="+", default=[2.0, 8.0, 32.0])
     ap.add_argument("--serve-queries", type=int, default=32)
     ap.add_argument("--budget-gb", type=float, default=200.0)
+    ap.add_argument("--gather-ceiling", action="store_true",
+                    help="time independent random 4-B reads (the ceiling of the grow round's access pattern)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device once (HBM counter calibration)")
     args = ap.parse_args()
     if args.calibrate_copy:
         return calibrate_copy(args.calibrate_copy)
+    if args.gather_ceiling:
+        return gather_ceiling()
     if args.serve and args.gpus == 1:
         if args.workload == "cl3m":
             args.workload = "cl4m"
@@ -211,9 +248,11 @@ def main():
     grow_gbs = gb / gt / 1e9 if gt > 0 else 0.0
     grow_rounds = max(sum(r["grow_rounds"] for r in plain), 1)
     tm = sum(r["t_merge_s"] for r in plain)
+    arcs_gs = sum(r["grow_edges"] for r in plain) / gt / 1e9 if gt > 0 else 0.0
     print(f"# plain: {plain_qps:.2f} q/s p50 {np.median(lat):.2f} ms p99 {np.percentile(lat, 99):.2f} ms; "
           f"grow {grow_gbs:.1f} GB/s over {grow_rounds} rounds ({gb / grow_rounds / 1e6:.1f} MB, "
-          f"{gt / grow_rounds * 1e6:.0f} us per round); merge {tm * 1e3:.1f} ms ({time.time() - t0:.1f}s)",
+          f"{gt / grow_rounds * 1e6:.0f} us per round, {arcs_gs:.1f} G arcs/s); merge {tm * 1e3:.1f} ms "
+          f"({time.time() - t0:.1f}s)",
           file=sys.stderr, flush=True)
     if args.profile_only:
         eng.close()
@@ -306,7 +345,10 @@ def main():
                      "kernel": "grow round: k_select + k_grow_thread + k_grow_wave (plain path, HIP events "
                                "on the engine stream)",
                      "launches": grow_rounds, "avg_launch_us": round(gt / grow_rounds * 1e6, 1),
-                     "bytes_per_launch": int(gb / grow_rounds)},
+                     "bytes_per_launch": int(gb / grow_rounds),
+                     # arcs scanned per second (one random target-state read each), to set
+                     # against bench.py --gather-ceiling
+                     "arcs_g_per_s": round(arcs_gs, 2)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
