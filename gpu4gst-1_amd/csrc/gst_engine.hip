@@ -103,6 +103,8 @@ struct gst_ctx {
   Plan *d_plan = nullptr;    // work layout of the single-query path
   unsigned long long *d_count = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;  // bracket the work kernels of one round
+  unsigned merge_grid = 1024;               // resident blocks of k_merge on this device
+  unsigned merge_hop_grid = 1024;           // resident blocks of k_merge_hop on this device
   int max_batch = 64;
   // batched / served (gst_batch.inc)
   struct Batch;
@@ -241,8 +243,14 @@ void fill_jobconst(const gst_ctx *c, const Slot &s, const Query &q, JobConst &jc
   long long members = 0;
   for (int i = 0; i < q.K; i++) members += c->h_goffs[q.gids[i] + 1] - c->h_goffs[q.gids[i]];
   jc.seed_total = (int32_t)members;
-  jc.tv = jc.hop ? std::max(1, std::min({256, HMERGE_LDS_WORDS / (jc.H * jc.W), HMERGE_MASKS / jc.H}))
-                 : std::min(512, MERGE_LDS_WORDS / jc.W);
+  const int ws = merge_row_stride(jc.W);
+  // hop tiles also keep each level segment (tv * H * C(k,l) words) within the SUH loads
+  // per lane that carry the next tile in registers
+  int cmax = 1;
+  for (int l = 1, cb = q.K; l <= q.K; cb = cb * (q.K - l) / (l + 1), l++) cmax = std::max(cmax, cb);
+  jc.tv = jc.hop ? std::max(1, std::min({256, HMERGE_LDS_WORDS / (jc.H * ws), HMERGE_MASKS / jc.H,
+                                         SUH * BLK / (jc.H * cmax)}))
+                 : std::min(512, MERGE_LDS_WORDS / ws);
   int cnt[8] = {0};
   for (int p = 0; p < jc.W; p++) {
     const int L = __builtin_popcount(p);
@@ -496,6 +504,16 @@ int gst_open(const gst_csr *csr, const gst_groups *groups, const gst_opts *opts,
   if (c->dev < 0 || c->dev >= ndev) { set_err("bad device ordinal"); return GST_BAD_INPUT; }
   HIPCHK(hipSetDevice(c->dev));
   HIPCHK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+  {
+    // the merges walk their tiles with a grid-stride loop: a grid of exactly the
+    // resident blocks leaves no partial second wave of blocks behind the first
+    int ncu = 0, nb = 0;
+    HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->dev));
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_merge_hop, BLK, 0));
+    c->merge_hop_grid = (unsigned)std::max(1, ncu * nb);
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_merge, BLK, 0));
+    c->merge_grid = (unsigned)std::max(1, ncu * nb);
+  }
   c->n = n;
   c->m = m;
   c->n_groups = groups->n_groups;
@@ -616,7 +634,6 @@ int gst_solve(gst_ctx *c, const int32_t *gids, int32_t k, int32_t D, int32_t fla
   JobState &h = c->h_js[0];
   double t_dp = 0;
   int guard = 0;
-  const unsigned merge_grid = 2048;
   bool dp_round = false;  // previous round was part of the DP loop
   int timed_phase = -1;   // phase whose work kernels ev0/ev1 bracket
   double t_grow = 0, t_merge = 0;
@@ -667,8 +684,8 @@ int gst_solve(gst_ctx *c, const int32_t *gids, int32_t k, int32_t D, int32_t fla
       case PH_MERGE: {
         const long long tiles = ((long long)c->n + jc.tv - 1) / jc.tv;
         HIPCHK(hipEventRecord(c->ev0, c->st));
-        if (q.D >= 0) k_merge_hop<<<grid_for(tiles, 1, merge_grid), BLK, 0, c->st>>>(c->d_jc, c->d_js, pl, g);
-        else k_merge<<<grid_for(tiles, 1, merge_grid), BLK, 0, c->st>>>(c->d_jc, c->d_js, pl, g);
+        if (q.D >= 0) k_merge_hop<<<grid_for(tiles, 1, c->merge_hop_grid), BLK, 0, c->st>>>(c->d_jc, c->d_js, pl, g);
+        else k_merge<<<grid_for(tiles, 1, c->merge_grid), BLK, 0, c->st>>>(c->d_jc, c->d_js, pl, g);
         if (h.level < q.K && (q.D != 0)) {
           const long long nblk = ((long long)c->n + 63) / 64 * (q.D > 0 ? q.D : 1);
           k_collect<<<grid_for(nblk, BLK / WAVE, GROW_GRID_CAP), BLK, 0, c->st>>>(c->d_jc, c->d_js, pl, g);

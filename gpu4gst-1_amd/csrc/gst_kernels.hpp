@@ -151,6 +151,15 @@ constexpr int JF_NO_TREE = 2;   // == GST_F_NO_TREE
 
 __device__ __forceinline__ int popc(uint32_t x) { return __builtin_popcount(x); }
 
+// Device memory through a global-address-space pointer: an access through a generic
+// pointer is a flat access, which also counts against the LDS/scalar counter, so every
+// later LDS or scalar wait would wait for it as well.
+typedef __attribute__((address_space(1))) int32_t gi32;
+typedef __attribute__((address_space(1))) unsigned long long gu64;
+__device__ __forceinline__ gi32 *as_global(int32_t *p) { return (gi32 *)p; }
+__device__ __forceinline__ const gi32 *as_global(const int32_t *p) { return (const gi32 *)p; }
+__device__ __forceinline__ gu64 *as_global(unsigned long long *p) { return (gu64 *)p; }
+
 // Arena index of state (v, p, d).
 __device__ __forceinline__ size_t sidx(const JobConst &c, int32_t v, int p, int d) {
   const int L = popc(p);
@@ -330,7 +339,7 @@ hop_entry: {
 
 // One block (BLK >= njobs threads): advance every job, then lay out this round's work
 // items per kind as exclusive prefix sums over jobs.
-__global__ __launch_bounds__(BLK) void k_step(const JobConst *jcs, JobState *jss, Plan *pl,
+__global__ __launch_bounds__(BLK) void k_step(const JobConst *__restrict__ jcs, JobState *jss, Plan *pl,
                                               int njobs, int n) {
   __shared__ long long cnt[NWK][MAXJOBS];
   const int j = threadIdx.x;
@@ -401,7 +410,7 @@ __global__ void k_fill_u64(unsigned long long *p, long long n, unsigned long lon
 
 // --------------------------------------------------------------------------- seed
 // T(x, {i}) = 0 for every member x of query group i (height 0); activates bit i.
-__global__ __launch_bounds__(BLK) void k_seed(const JobConst *jcs, JobState *jss, const Plan *pl, GraphDev g) {
+__global__ __launch_bounds__(BLK) void k_seed(const JobConst *__restrict__ jcs, JobState *jss, const Plan *pl, GraphDev g) {
   __shared__ long long soff[MAXJOBS + 1];
   const int nj = load_plan(pl, WK_SEED, soff);
   const long long total = soff[nj];
@@ -557,7 +566,8 @@ struct RelaxGroup {
 };
 
 // src = the source vertex's level row
-__device__ __forceinline__ void take_group(const int32_t *src, unsigned long long &gm, RelaxGroup &G) {
+__device__ __forceinline__ void take_group(const int32_t *srcp, unsigned long long &gm, RelaxGroup &G) {
+  const gi32 *src = as_global(srcp);
   G.n = 0;
 #pragma unroll
   for (int i = 0; i < RG; i++) {
@@ -706,12 +716,12 @@ __device__ void wp_flush(WavePush &b, const JobConst *jcs, JobState *jss, const 
     if ((lane == 0 && b.ns) || (lane == 1 && b.nb)) r = atomicAdd(&s->cnt[b.lo][lane], lane ? tot : b.ns);
     const int base_s = __shfl(r, 0, 64), base_b = __shfl(r, 1, 64);
     if (b.ns) {
-      int32_t *dst = c.lst + (size_t)b.lo * g.n + base_s;
+      gi32 *dst = as_global(c.lst + (size_t)b.lo * g.n + base_s);
       for (int i = lane; i < b.ns; i += WAVE) dst[i] = b.sm[i];
     }
     if (lane < b.nb) {
       const int32_t v = b.bv[lane];
-      int32_t *dst = c.blst + ((size_t)b.lo * c.bcap + base_b + x - nch) * 2;
+      gi32 *dst = as_global(c.blst + ((size_t)b.lo * c.bcap + base_b + x - nch) * 2);
       for (int i = 0; i < nch; i++) { dst[2 * i] = v; dst[2 * i + 1] = i; }
     }
   }
@@ -873,11 +883,12 @@ __device__ __forceinline__ void relax_edges(int32_t *dbase, size_t dstride, unsi
                                             const GraphDev &g, const RelaxGroup &G, const bool (&h)[NE],
                                             const int32_t (&u)[NE], const int32_t (&w)[NE], bool (&want)[NE],
                                             int (&nch)[NE], int &n_atom) {
-  int32_t *Td[NE];
+  gi32 *Td[NE];
+  gu64 *const act = as_global(act_lo);
   int t[NE][RG];
 #pragma unroll
   for (int j = 0; j < NE; j++) {
-    Td[j] = dbase + (size_t)u[j] * dstride;
+    Td[j] = as_global(dbase + (size_t)u[j] * dstride);
 #pragma unroll
     for (int i = 0; i < RG; i++) t[j][i] = (h[j] && i < G.n) ? Td[j][G.p[i]] : 0;
   }
@@ -890,7 +901,11 @@ __device__ __forceinline__ void relax_edges(int32_t *dbase, size_t dstride, unsi
       if (h[j] && i < G.n) {
         int nc = G.c[i] + w[j];
         nc = nc < INF ? nc : INF;
-        if (nc < t[j][i]) { atomicMin(&Td[j][G.p[i]], nc); n_atom++; imp[j] |= 1ull << G.p[i]; }
+        if (nc < t[j][i]) {
+          __hip_atomic_fetch_min(&Td[j][G.p[i]], nc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          n_atom++;
+          imp[j] |= 1ull << G.p[i];
+        }
       }
     }
   }
@@ -898,9 +913,12 @@ __device__ __forceinline__ void relax_edges(int32_t *dbase, size_t dstride, unsi
   // per-edge load would fetch another random line
   unsigned long long cur[NE];
 #pragma unroll
-  for (int j = 0; j < NE; j++) cur[j] = imp[j] ? act_lo[u[j]] : ~0ull;
+  for (int j = 0; j < NE; j++) cur[j] = imp[j] ? act[u[j]] : ~0ull;
 #pragma unroll
-  for (int j = 0; j < NE; j++) want[j] = (imp[j] & ~cur[j]) ? atomicOr(&act_lo[u[j]], imp[j]) == 0 : false;
+  for (int j = 0; j < NE; j++)
+    want[j] = (imp[j] & ~cur[j])
+                  ? __hip_atomic_fetch_or(&act[u[j]], imp[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0
+                  : false;
 #pragma unroll
   for (int j = 0; j < NE; j++) nch[j] = want[j] ? chunks_of(g, u[j]) : 0;
 }
@@ -923,7 +941,7 @@ struct ThreadSeg {
   long long dstride[WAVE];
 };
 
-__global__ __launch_bounds__(BLK) void k_grow_thread(const JobConst *jcs, JobState *jss, const Plan *pl, GraphDev g) {
+__global__ __launch_bounds__(BLK) void k_grow_thread(const JobConst *__restrict__ jcs, JobState *jss, const Plan *pl, GraphDev g) {
   __shared__ long long soff[MAXJOBS + 1];
   __shared__ int32_t pb_small[(BLK / WAVE) * PB_SMALL];
   __shared__ int32_t pb_big[(BLK / WAVE) * 2 * PB_BIG];
@@ -1030,7 +1048,7 @@ __global__ __launch_bounds__(BLK) void k_grow_thread(const JobConst *jcs, JobSta
 // together: most arcs of this kernel sit in chunks of 512-1024 edges, which then take
 // 2-4 dependent round-trip chains instead of 4-8.
 constexpr int GW_NE = 4;
-__global__ __launch_bounds__(BLK) void k_grow_wave(const JobConst *jcs, JobState *jss, const Plan *pl, GraphDev g) {
+__global__ __launch_bounds__(BLK) void k_grow_wave(const JobConst *__restrict__ jcs, JobState *jss, const Plan *pl, GraphDev g) {
   __shared__ long long soff[MAXJOBS + 1];
   __shared__ int32_t pb_small[(BLK / WAVE) * PB_SMALL];
   __shared__ int32_t pb_big[(BLK / WAVE) * 2 * PB_BIG];
@@ -1046,6 +1064,11 @@ __global__ __launch_bounds__(BLK) void k_grow_wave(const JobConst *jcs, JobState
   const int32_t *__restrict__ wts = g.w;
   const long long waves_per_grid = (long long)gridDim.x * (BLK / WAVE);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: scalar registers
+  // fields of job cj (set when cj changes)
+  int jlo = 0, jL = 0, jd = 0;
+  long long jrow = 0;
+  int32_t *jsrc = nullptr, *jdst = nullptr;
+  unsigned long long *jact = nullptr;
   // The wave's items are it0, it0 + S, it0 + 2S, ... (S = waves in the grid). Each lane
   // fetches the header of one of the next 64 (vertex, chunk, active mask, arc range, job),
   // so the header loads of 64 items share one memory latency; the wave then walks them.
@@ -1057,10 +1080,10 @@ __global__ __launch_bounds__(BLK) void k_grow_wave(const JobConst *jcs, JobState
       hjb = find_job(soff, nj, my);
       const JobConst &c = jcs[hjb];
       const int li = cur_list(c, &jss[hjb]);
-      const int32_t *item = c.blst + ((size_t)li * c.bcap + (my - soff[hjb])) * 2;
+      const gi32 *item = as_global(c.blst + ((size_t)li * c.bcap + (my - soff[hjb])) * 2);
       hv = item[0];
       hch = item[1];
-      hgm = c.act[(size_t)li * g.n + hv];  // already reduced by k_select
+      hgm = as_global(c.act)[(size_t)li * g.n + hv];  // already reduced by k_select
       heb = g.beg[hv] + hch * g.chunk;
       hee = min(g.beg[hv + 1], heb + g.chunk);
     }
@@ -1068,19 +1091,30 @@ __global__ __launch_bounds__(BLK) void k_grow_wave(const JobConst *jcs, JobState
     const int nitems = (int)(left < WAVE ? left : WAVE);
     for (int l = 0; l < nitems; l++) {
     const int jb = __builtin_amdgcn_readlane(hjb, l);
-    if (jb != cj) { flush_lane(jcs, jss, cj, k); k.zero(); cj = jb; }
-    const JobConst &c = jcs[jb];
-    JobState *s = &jss[jb];
-    const int lo = nxt_list(c, s);
-    const int L = s->level;
-    const int d = c.hop ? s->depth : 0;
-    const size_t dstride = (size_t)c.H * c.nsub[L];
-    int32_t *const dbase = lrow(c, L, 0, c.hop ? d + 1 : 0);
-    unsigned long long *const act_lo = c.act + (size_t)lo * g.n;
+    if (jb != cj) {
+      // the job's list, level and rows: read once per job, not per item (every read of
+      // the job state is a dependent round trip ahead of the item's loads)
+      flush_lane(jcs, jss, cj, k);
+      k.zero();
+      cj = jb;
+      const JobConst &c = jcs[jb];
+      const JobState *s = &jss[jb];
+      jlo = __builtin_amdgcn_readfirstlane(nxt_list(c, s));
+      jL = __builtin_amdgcn_readfirstlane(s->level);
+      jd = c.hop ? __builtin_amdgcn_readfirstlane(s->depth) : 0;
+      jrow = (long long)c.H * c.nsub[jL];
+      jsrc = lrow(c, jL, 0, jd);
+      jdst = lrow(c, jL, 0, c.hop ? jd + 1 : 0);
+      jact = c.act + (size_t)jlo * g.n;
+    }
+    const int lo = jlo;
+    const size_t dstride = (size_t)jrow;
+    int32_t *const dbase = jdst;
+    unsigned long long *const act_lo = jact;
     const int32_t v = __builtin_amdgcn_readlane(hv, l), ch = __builtin_amdgcn_readlane(hch, l);
     unsigned long long gm = ((unsigned long long)(uint32_t)__builtin_amdgcn_readlane((int)(hgm >> 32), l) << 32) |
                             (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)hgm, l);
-    int32_t *src = lrow(c, L, v, d);
+    const int32_t *src = jsrc + (size_t)v * dstride;
     if (lane == 0 && ch == 0) k.verts++;
     if (!gm) continue;
     const int nb = popc((uint32_t)gm) + popc((uint32_t)(gm >> 32));
@@ -1121,7 +1155,7 @@ __global__ __launch_bounds__(BLK) void k_grow_wave(const JobConst *jcs, JobState
 
 // Zero the active masks of the list just consumed (after both grow kernels ran), or of
 // the height-D list at the end of a hop level.
-__global__ __launch_bounds__(BLK) void k_clear(const JobConst *jcs, JobState *jss, const Plan *pl, GraphDev g) {
+__global__ __launch_bounds__(BLK) void k_clear(const JobConst *__restrict__ jcs, JobState *jss, const Plan *pl, GraphDev g) {
   __shared__ long long soff[MAXJOBS + 1];
   const int nj = load_plan(pl, WK_CLEAR, soff);
   const long long total = soff[nj];
@@ -1141,7 +1175,7 @@ __global__ __launch_bounds__(BLK) void k_clear(const JobConst *jcs, JobState *js
 // After level 1: r1 = UB/2 + 1, group distance lower bounds vv[a][b] and Held-Karp tour
 // tables hk[a][b][X] (cheapest walk from group a through every group of X ending at
 // b in X) and w2[a][X] = min_b hk[a][b][X]. One block per job.
-__global__ __launch_bounds__(BLK) void k_tables(const JobConst *jcs, JobState *jss, int njobs, GraphDev g) {
+__global__ __launch_bounds__(BLK) void k_tables(const JobConst *__restrict__ jcs, JobState *jss, int njobs, GraphDev g) {
   if ((int)blockIdx.x >= njobs) return;
   const JobConst &c = jcs[blockIdx.x];
   JobState *s = &jss[blockIdx.x];
@@ -1214,16 +1248,22 @@ struct UDiv {
   __device__ __forceinline__ uint32_t div(uint32_t x) const { return d == 1 ? x : __umulhi(x, m); }
 };
 
+// LDS rows of the merge tiles are W + 1 words apart: with a stride of W (a multiple of
+// the bank count for k >= 5) the lanes that read the same subset p of different rows
+// all hit one bank, up to a 32-way conflict at the top levels.
+__host__ __device__ constexpr int merge_row_stride(int W) { return W + 1; }
+
 // Stages levels 1..L-1 of the tile's vertices [v0, v0 + tv) into LDS as
-// rows[(vi * H + d) * W + p]. Each level is one contiguous segment of its level block
-// whose word i is row i / C(k,l), subset rank i % C(k,l); a segment holds
-// tv * H * C(k,l) <= 4096 * C(k,l) / W <= 2048 = SU * BLK words (C(k,l) <= 2^(k-1)), so
-// every lane issues the loads of all levels (at most (MAXK-1) * SU) before the first LDS
-// store: one memory latency per tile. lsub is the job's subset table, copied to LDS.
+// rows[(vi * H + d) * WS + p], WS = merge_row_stride(W). Each level is one contiguous
+// segment of its level block whose word i is row i / C(k,l), subset rank i % C(k,l); a
+// segment holds tv * H * C(k,l) <= 4160 * C(k,l) / (W + 1) < 2048 = SU * BLK words
+// (C(k,l) <= 2^(k-1)), so every lane issues the loads of all levels (at most
+// (MAXK-1) * SU) before the first LDS store: one memory latency per tile. lsub is the
+// job's subset table, copied to LDS.
 constexpr int SU = 8;
 __device__ __forceinline__ void stage_levels(const JobConst &c, int L, long long v0, int tv, int H,
                                              const uint8_t (*lsub)[64], int32_t *rows) {
-  const int W = c.W;
+  const int WS = merge_row_stride(c.W);
   const int t = (int)threadIdx.x;
   int32_t x[MAXK - 1][SU];
 #pragma unroll
@@ -1249,7 +1289,7 @@ __device__ __forceinline__ void stage_levels(const JobConst &c, int L, long long
         const int i = u * BLK + t;
         if (i < n) {
           const int r = (int)dn.div((uint32_t)i);
-          rows[r * W + lsub[l][i - r * ns]] = x[l - 1][u];
+          rows[r * WS + lsub[l][i - r * ns]] = x[l - 1][u];
         }
       }
     }
@@ -1280,8 +1320,8 @@ __device__ __forceinline__ void flush_merge(const JobConst *jcs, JobState *jss, 
 // Non-hop merge for level L: T(v,p) = min over splits p = p1 + p2 of T(v,p1) + T(v,p2).
 // A tile of vertices has its rows staged in LDS (coalesced contiguous read), each
 // thread evaluates (vertex, subset) items, new states are activated for growth.
-constexpr int MERGE_LDS_WORDS = 4096;
-__global__ __launch_bounds__(BLK) void k_merge(const JobConst *jcs, JobState *jss, const Plan *pl, GraphDev g) {
+constexpr int MERGE_LDS_WORDS = 4160;  // 32 rows of 129 words (k = 7), 126 of 33 (k = 5)
+__global__ __launch_bounds__(BLK) void k_merge(const JobConst *__restrict__ jcs, JobState *jss, const Plan *pl, GraphDev g) {
   __shared__ long long soff[MAXJOBS + 1];
   __shared__ int32_t rows[MERGE_LDS_WORDS];
   __shared__ unsigned long long amask[512];
@@ -1323,7 +1363,7 @@ __global__ __launch_bounds__(BLK) void k_merge(const JobConst *jcs, JobState *js
     for (int it = threadIdx.x; it < tv * nL; it += BLK) {
       const int vi = (int)dnl.div((uint32_t)it), j = it - vi * nL;
       const int p = lsub[L][j];
-      const int32_t *row = rows + vi * W;
+      const int32_t *row = rows + vi * merge_row_stride(W);
       const int low = p & -p, rest = p ^ low;
       int m = INF;
       for (int sm = rest;; sm = (sm - 1) & rest) {
@@ -1376,62 +1416,130 @@ __global__ __launch_bounds__(BLK) void k_merge(const JobConst *jcs, JobState *js
 // the best pair of heights (d1, d2) with max(d1, d2) <= nd and d1 + d2 <= D. The minimum
 // over heights of the full states, the answer, is the same as with exact heights.
 // Splits whose halves have no finite state at the vertex are skipped (finm).
-constexpr int HMERGE_LDS_WORDS = 4096;
-constexpr int HMERGE_MASKS = HMERGE_LDS_WORDS / 8;   // tv * H <= 4096 / W for W >= 8; k = 2 tiles are clamped
-__global__ __launch_bounds__(BLK) void k_merge_hop(const JobConst *jcs, JobState *jss, const Plan *pl, GraphDev g) {
+constexpr int HMERGE_LDS_WORDS = 4160;  // e.g. 5 vertices x 6 heights x 129 words (k = 7, D = 5)
+constexpr int HMERGE_MASKS = 512;       // tv * H <= 4160 / (W + 1) for W >= 8; k = 2 tiles are clamped
+// Registers that carry the next tile's rows while the block evaluates the current one:
+// a hop tile's level segment holds at most 1464 words (k = 4, D = 0, level 2), so SUH
+// loads per lane and level cover it.
+constexpr int SUH = 6;
+
+// Loads levels 1..L-1 of the rows of vertices [v0, v0 + tv) of job c into x (word i of
+// level l's contiguous segment in x[l - 1][i / BLK] of thread i % BLK), without waiting.
+__device__ __forceinline__ void load_hop_tile(const JobConst &c, int L, long long v0, int tv,
+                                              int32_t (&x)[MAXK - 1][SUH]) {
+  const int t = (int)threadIdx.x;
+  // the subset counts as one 8-byte scalar load
+  const unsigned long long nsw = *(const unsigned long long *)c.nsub;
+#pragma unroll
+  for (int l = 1; l < MAXK; l++) {
+    if (l < L) {
+      const int ns = (int)((nsw >> (8 * l)) & 0xff), n = tv * c.H * ns;
+      const gi32 *src = as_global(c.T + c.off[l] + v0 * c.H * ns);
+      // every lane loads (past the segment end: its last word, never stored): a
+      // register written by an ALU op while an older load into it may be pending would
+      // make the compiler wait for all loads in flight, the previous level's included
+#pragma unroll
+      for (int u = 0; u < SUH; u++) x[l - 1][u] = src[min(u * BLK + t, n - 1)];
+    }
+  }
+}
+
+// Issues the loads of merge tile gt into x if it belongs to a hop job; returns gt, or -1.
+// The job index is made uniform and the job table is read-only in the kernel, so the
+// job's fields come through scalar loads and never wait behind the loads in flight.
+// lvl: each job's level, in LDS.
+__device__ __forceinline__ long long issue_hop_tile(const JobConst *__restrict__ jcs, const int *lvl,
+                                                    const long long *soff, int nj, long long gt, long long total,
+                                                    int n, int32_t (&x)[MAXK - 1][SUH]) {
+  if (gt >= total) return -1;
+  const int jb = __builtin_amdgcn_readfirstlane(find_job(soff, nj, gt));
+  const JobConst &c = jcs[jb];
+  if (!c.hop) return -1;
+  const long long v0 = (gt - soff[jb]) * c.tv;
+  load_hop_tile(c, lvl[jb], v0, (int)min((long long)c.tv, (long long)n - v0), x);
+  return gt;
+}
+
+__global__ __launch_bounds__(BLK) void k_merge_hop(const JobConst *__restrict__ jcs, JobState *jss, const Plan *pl,
+                                                   GraphDev g) {
   __shared__ long long soff[MAXJOBS + 1];
+  __shared__ int lvl[MAXJOBS];
   __shared__ int32_t pm[HMERGE_LDS_WORDS];
   __shared__ unsigned long long amask[HMERGE_MASKS];
   __shared__ unsigned long long finm[256][2];  // per tile vertex: subsets with a finite state
-  __shared__ int ubs;
+  __shared__ int ubs, ub0s;
   __shared__ uint32_t lsubw[128];
   uint8_t (*lsub)[64] = (uint8_t (*)[64])lsubw;
-  const int nj = load_plan(pl, WK_MERGE, soff);
+  for (int j = threadIdx.x; j < pl->njobs; j += BLK) lvl[j] = jss[j].level;
+  const int nj = load_plan(pl, WK_MERGE, soff);  // (its barrier also publishes lvl)
   const long long total = soff[nj];
   int cj = -1, minc_acc = INF;
   unsigned long long merged_acc = 0;
+  // A tile's rows travel in registers while the block evaluates the previous tile: the
+  // loads of tile gt + gridDim.x are issued once tile gt is in LDS.
+  int32_t x[MAXK - 1][SUH];
+  long long issued = issue_hop_tile(jcs, lvl, soff, nj, blockIdx.x, total, g.n, x);
   for (long long gt = blockIdx.x; gt < total; gt += gridDim.x) {
-  const int jb = find_job(soff, nj, gt);
+  const int jb = __builtin_amdgcn_readfirstlane(find_job(soff, nj, gt));
   if (jb != cj) { flush_merge(jcs, jss, cj, merged_acc, minc_acc); cj = jb; }
   const JobConst &c = jcs[jb];
   JobState *s = &jss[jb];
   if (!c.hop) continue;
-  const int W = c.W, H = c.H, D = c.D, K = c.K, L = s->level;
-  const int RW = H * W;
+  const int W = c.W, H = c.H, D = c.D, K = c.K, L = lvl[jb];
+  const int WS = merge_row_stride(W), RW = H * WS;
   const int TV = c.tv;
   const int nL = c.nsub[L];
   int merged = 0;
   {
     const long long v0 = (gt - soff[jb]) * TV;
     const int tv = (int)min((long long)TV, (long long)g.n - v0);
+    if (issued != gt) issued = issue_hop_tile(jcs, lvl, soff, nj, gt, total, g.n, x);
     for (int i = threadIdx.x; i < tv * H; i += BLK) amask[i] = 0;
     for (int i = threadIdx.x; i < 2 * tv; i += BLK) finm[i >> 1][i & 1] = 0;
-    if (threadIdx.x == 0) ubs = INF;
-    // UB known when the tile starts (its load overlaps the staging): UB candidates of the
-    // tile are compared with it, and the growth filter of the hop select pass uses it
-    const int ub0 = load_ub(s);
-    // every subset read below has fewer than L members, so staging overwrites all of it
+    // UB known when the tile starts: UB candidates of the tile are compared with it, and
+    // the growth filter of the hop select pass uses it. Read by one thread and shared
+    // through LDS, so that no vector load is pending while the next tile's rows are.
+    if (threadIdx.x == 0) { ubs = INF; ub0s = load_ub(s); }
     load_lsub(c, lsub);
-    stage_levels(c, L, v0, tv, H, lsub, pm);
+    __syncthreads();
+    const int ub0 = ub0s;
+    // rows -> pm[(vi * H + d) * WS + q]; every subset read below has fewer than L members,
+    // so this overwrites all of it
+#pragma unroll
+    for (int l = 1; l < MAXK; l++) {
+      if (l < L) {
+        const int ns = c.nsub[l], n = tv * H * ns;
+        const UDiv dn(ns);
+#pragma unroll
+        for (int u = 0; u < SUH; u++) {
+          const int i = u * BLK + (int)threadIdx.x;
+          if (i < n) {
+            const int r = (int)dn.div((uint32_t)i);
+            pm[r * WS + lsub[l][i - r * ns]] = x[l - 1][u];
+          }
+        }
+      }
+    }
     __syncthreads();
     for (int i = threadIdx.x; i < tv * W; i += BLK) {  // prefix minima over height, in place
       const int vi = i >> c.K, q = i & (W - 1);
       if (q == 0 || popc(q) >= L) continue;
       int m = INF;
       for (int d = 0; d < H; d++) {
-        int32_t *x = &pm[vi * RW + d * W + q];
-        m = *x < m ? *x : m;
-        *x = m;
+        int32_t *y = &pm[vi * RW + d * WS + q];
+        m = *y < m ? *y : m;
+        *y = m;
       }
       if (m < INF) atomicOr(&finm[vi][q >> 6], 1ull << (q & 63));
     }
+    issued = issue_hop_tile(jcs, lvl, soff, nj, gt + gridDim.x, total, g.n, x);
     __syncthreads();
     const bool prune = !(c.flags & JF_NO_PRUNE);
     int ubl = INF;
     if (L == 2) {  // stars: arms of at most floor(D/2) hops, prefix minima of the singletons
       for (int vi = threadIdx.x; vi < tv; vi += BLK) {
         long long star = 0;
-        for (int i = 0; i < K; i++) star += pm[(vi * H + D / 2) * W + (1 << i)];
+        for (int i = 0; i < K; i++) star += pm[(vi * H + D / 2) * WS + (1 << i)];
         if (star < ubl) ubl = (int)star;
       }
     }
@@ -1445,8 +1553,8 @@ __global__ __launch_bounds__(BLK) void k_merge_hop(const JobConst *jcs, JobState
       const unsigned long long f0 = finm[vi][0], f1 = finm[vi][1];
       const int low = p & -p, rest = p ^ low;
       const int b = min(nd, D - nd);
-      const int32_t *pn = pm + vi * RW + nd * W;  // heights <= nd
-      const int32_t *pb = pm + vi * RW + b * W;   // heights <= b
+      const int32_t *pn = pm + vi * RW + nd * WS;  // heights <= nd
+      const int32_t *pb = pm + vi * RW + b * WS;   // heights <= b
       int m = INF;
       for (int sm = rest;; sm = (sm - 1) & rest) {
         const int p1 = low | sm, p2 = p ^ p1;
@@ -1464,7 +1572,7 @@ __global__ __launch_bounds__(BLK) void k_merge_hop(const JobConst *jcs, JobState
       }
       if (m >= INF) continue;
       const int q = c.R ^ p;
-      const int32_t *pr = pm + (vi * H + (D - nd)) * W;  // minima over heights <= D - nd
+      const int32_t *pr = pm + (vi * H + (D - nd)) * WS;  // minima over heights <= D - nd
       if (q == 0) ubl = m < ubl ? m : ubl;
       else if (popc(q) < L && (long long)m + pr[q] < ubl) ubl = m + pr[q];
       if (prune && L < K) {
@@ -1476,7 +1584,7 @@ __global__ __launch_bounds__(BLK) void k_merge_hop(const JobConst *jcs, JobState
         if ((long long)m + lb > ub0) continue;
       }
       merged++;
-      c.T[(size_t)c.off[L] + (size_t)v0 * H * nL + it] = m;
+      as_global(c.T)[(size_t)c.off[L] + (size_t)v0 * H * nL + it] = m;
       if (L < K && nd < D) atomicOr(&amask[vi * H + nd], 1ull << j);
     }
     if (ubl < ub0) atomicMin(&ubs, ubl);
@@ -1486,7 +1594,7 @@ __global__ __launch_bounds__(BLK) void k_merge_hop(const JobConst *jcs, JobState
       const unsigned long long mk = amask[i];
       if (mk) {
         const int vi = (int)dh.div((uint32_t)i), nd = i - vi * H;
-        c.act[(size_t)nd * g.n + v0 + vi] = mk;
+        as_global(c.act)[(size_t)nd * g.n + v0 + vi] = mk;
       }
     }
     __syncthreads();
@@ -1502,7 +1610,7 @@ __global__ __launch_bounds__(BLK) void k_merge_hop(const JobConst *jcs, JobState
 // list-counter round trips are off the merge tiles' critical path. Masks outside the
 // lists are zero when a merge starts (consumed masks are cleared every round), so a
 // nonzero mask is exactly a merge activation.
-__global__ __launch_bounds__(BLK) void k_collect(const JobConst *jcs, JobState *jss, const Plan *pl, GraphDev g) {
+__global__ __launch_bounds__(BLK) void k_collect(const JobConst *__restrict__ jcs, JobState *jss, const Plan *pl, GraphDev g) {
   __shared__ long long soff[MAXJOBS + 1];
   __shared__ int32_t pb_small[(BLK / WAVE) * PB_SMALL];
   __shared__ int32_t pb_big[(BLK / WAVE) * 2 * PB_BIG];
@@ -1530,7 +1638,7 @@ __global__ __launch_bounds__(BLK) void k_collect(const JobConst *jcs, JobState *
 // --------------------------------------------------------------------------- final
 // ans = min over v, d of (T(v, full, d) << 32 | v*H + d); a work item is a chunk of
 // FINAL_CHUNK rows (v, d) of one job.
-__global__ __launch_bounds__(BLK) void k_final(const JobConst *jcs, JobState *jss, const Plan *pl, GraphDev g) {
+__global__ __launch_bounds__(BLK) void k_final(const JobConst *__restrict__ jcs, JobState *jss, const Plan *pl, GraphDev g) {
   __shared__ long long soff[MAXJOBS + 1];
   __shared__ unsigned long long bm;
   const int nj = load_plan(pl, WK_FINAL, soff);
