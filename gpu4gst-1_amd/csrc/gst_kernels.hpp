@@ -159,6 +159,7 @@ typedef __attribute__((address_space(1))) unsigned long long gu64;
 __device__ __forceinline__ gi32 *as_global(int32_t *p) { return (gi32 *)p; }
 __device__ __forceinline__ const gi32 *as_global(const int32_t *p) { return (const gi32 *)p; }
 __device__ __forceinline__ gu64 *as_global(unsigned long long *p) { return (gu64 *)p; }
+__device__ __forceinline__ const gu64 *as_global(const unsigned long long *p) { return (const gu64 *)p; }
 
 // Arena index of state (v, p, d).
 __device__ __forceinline__ size_t sidx(const JobConst &c, int32_t v, int p, int d) {
@@ -958,22 +959,66 @@ __global__ __launch_bounds__(BLK) void k_grow_thread(const JobConst *__restrict_
   const long long stride = (long long)gridDim.x * BLK;
   const int32_t *__restrict__ adj = g.adj;
   const int32_t *__restrict__ wts = g.w;
+  // Fields of job ucj, wave-uniform: most waves hold items of one job only, and these
+  // are read once per job instead of once per pass (each read of the job state would be
+  // a dependent round trip ahead of the items' loads).
+  int ucj = -1, ulo = 0;
+  long long urow = 0;
+  const int32_t *ulst = nullptr, *usrc = nullptr;
+  int32_t *udst = nullptr;
+  const unsigned long long *uacur = nullptr;
+  unsigned long long *uanxt = nullptr;
   for (long long wbase = (long long)blockIdx.x * BLK + (threadIdx.x & ~63); wbase < total; wbase += stride) {
     const long long it = wbase + lane;
     int deg = 0;
     const int32_t *src = nullptr;
     unsigned long long gm = 0;
+    const int jf = __builtin_amdgcn_readfirstlane(find_job(soff, nj, wbase));
+    const long long wl = wbase + WAVE - 1 < total ? wbase + WAVE - 1 : total - 1;
+    const bool uni = jf == __builtin_amdgcn_readfirstlane(find_job(soff, nj, wl));
+    if (uni && jf != ucj) {
+      ucj = jf;
+      const JobConst &c = jcs[jf];
+      const JobState *s = &jss[jf];
+      const int li = __builtin_amdgcn_readfirstlane(cur_list(c, s));
+      ulo = __builtin_amdgcn_readfirstlane(nxt_list(c, s));
+      const int L = __builtin_amdgcn_readfirstlane(s->level);
+      const int d = c.hop ? __builtin_amdgcn_readfirstlane(s->depth) : 0;
+      urow = (long long)c.H * c.nsub[L];
+      ulst = c.lst + (size_t)li * g.n;
+      uacur = c.act + (size_t)li * g.n;
+      uanxt = c.act + (size_t)ulo * g.n;
+      usrc = lrow(c, L, 0, d);
+      udst = lrow(c, L, 0, c.hop ? d + 1 : 0);
+    }
     if (it < total) {
-      const int jb = find_job(soff, nj, it);
+      int jb, lo;
+      int32_t v;
+      if (uni) {
+        jb = jf;
+        lo = ulo;
+        v = as_global(ulst)[it - soff[jf]];
+        gm = as_global(uacur)[v];  // already reduced by k_select
+        src = usrc + (size_t)v * urow;
+        S.dbase[lane] = udst;
+        S.dstride[lane] = urow;
+        S.act[lane] = uanxt;
+      } else {  // a wave across two jobs: per-lane fields
+        jb = find_job(soff, nj, it);
+        const JobConst &c = jcs[jb];
+        const JobState *s = &jss[jb];
+        const int li = cur_list(c, s), L = s->level;
+        const int d = c.hop ? s->depth : 0;
+        lo = nxt_list(c, s);
+        v = c.lst[(size_t)li * g.n + (it - soff[jb])];
+        gm = c.act[(size_t)li * g.n + v];
+        src = lrow(c, L, v, d);
+        S.dbase[lane] = lrow(c, L, 0, c.hop ? d + 1 : 0);
+        S.dstride[lane] = (long long)c.H * c.nsub[L];
+        S.act[lane] = c.act + (size_t)lo * g.n;
+      }
       if (jb != cj) { flush_lane(jcs, jss, cj, k); k.zero(); cj = jb; }
-      const JobConst &c = jcs[jb];
-      JobState *s = &jss[jb];
-      const int li = cur_list(c, s), lo = nxt_list(c, s), L = s->level;
-      const int d = c.hop ? s->depth : 0;
-      const int32_t v = c.lst[(size_t)li * g.n + (it - soff[jb])];
-      gm = c.act[(size_t)li * g.n + v];  // already reduced by k_select
       k.verts++;
-      src = lrow(c, L, v, d);
       const int32_t e0 = g.beg[v];
       deg = g.beg[v + 1] - e0;
       if (gm && deg) {
@@ -986,9 +1031,6 @@ __global__ __launch_bounds__(BLK) void k_grow_thread(const JobConst *__restrict_
       S.e0[lane] = e0;
       S.jb[lane] = jb;
       S.lo[lane] = lo;
-      S.dbase[lane] = lrow(c, L, 0, c.hop ? d + 1 : 0);
-      S.dstride[lane] = (long long)c.H * c.nsub[L];
-      S.act[lane] = c.act + (size_t)lo * g.n;
     }
     for (;;) {  // one pass per group of up to RG active states of every item
       RelaxGroup G;
