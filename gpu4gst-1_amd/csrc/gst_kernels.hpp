@@ -1402,19 +1402,20 @@ __global__ __launch_bounds__(BLK) void k_merge(const JobConst *__restrict__ jcs,
     const int ub = ubs;
     int ubl = ub, mact = INF;
     const UDiv dnl(nL);
+    const int nsplit = 1 << (L - 1);  // splits per subset of the level, plus one
     for (int it = threadIdx.x; it < tv * nL; it += BLK) {
       const int vi = (int)dnl.div((uint32_t)it), j = it - vi * nL;
       const int p = lsub[L][j];
       const int32_t *row = rows + vi * merge_row_stride(W);
       const int low = p & -p, rest = p ^ low;
+      // splits p1 + p2 = p with low in p1: sm runs over the proper subsets of rest, the
+      // same count for every item of the level (a uniform loop)
       int m = INF;
-      for (int sm = rest;; sm = (sm - 1) & rest) {
+      int sm = rest;
+      for (int t = 1; t < nsplit; t++) {
+        sm = (sm - 1) & rest;
         const int p1 = low | sm;
-        if (p1 != p) {
-          const int t = row[p1] + row[p ^ p1];
-          m = t < m ? t : m;
-        }
-        if (sm == 0) break;
+        m = min(m, row[p1] + row[p ^ p1]);
       }
       if (m >= INF) continue;
       merged++;
@@ -1457,7 +1458,6 @@ __global__ __launch_bounds__(BLK) void k_merge(const JobConst *__restrict__ jcs,
 //               min(pm(nd,p1) + pm(b,p2), pm(b,p1) + pm(nd,p2)),  b = min(nd, D - nd),
 // the best pair of heights (d1, d2) with max(d1, d2) <= nd and d1 + d2 <= D. The minimum
 // over heights of the full states, the answer, is the same as with exact heights.
-// Splits whose halves have no finite state at the vertex are skipped (finm).
 constexpr int HMERGE_LDS_WORDS = 4160;  // e.g. 5 vertices x 6 heights x 129 words (k = 7, D = 5)
 constexpr int HMERGE_MASKS = 512;       // tv * H <= 4160 / (W + 1) for W >= 8; k = 2 tiles are clamped
 // Registers that carry the next tile's rows while the block evaluates the current one:
@@ -1508,7 +1508,6 @@ __global__ __launch_bounds__(BLK) void k_merge_hop(const JobConst *__restrict__ 
   __shared__ int lvl[MAXJOBS];
   __shared__ int32_t pm[HMERGE_LDS_WORDS];
   __shared__ unsigned long long amask[HMERGE_MASKS];
-  __shared__ unsigned long long finm[256][2];  // per tile vertex: subsets with a finite state
   __shared__ int ubs, ub0s;
   __shared__ uint32_t lsubw[128];
   uint8_t (*lsub)[64] = (uint8_t (*)[64])lsubw;
@@ -1537,7 +1536,6 @@ __global__ __launch_bounds__(BLK) void k_merge_hop(const JobConst *__restrict__ 
     const int tv = (int)min((long long)TV, (long long)g.n - v0);
     if (issued != gt) issued = issue_hop_tile(jcs, lvl, soff, nj, gt, total, g.n, x);
     for (int i = threadIdx.x; i < tv * H; i += BLK) amask[i] = 0;
-    for (int i = threadIdx.x; i < 2 * tv; i += BLK) finm[i >> 1][i & 1] = 0;
     // UB known when the tile starts: UB candidates of the tile are compared with it, and
     // the growth filter of the hop select pass uses it. Read by one thread and shared
     // through LDS, so that no vector load is pending while the next tile's rows are.
@@ -1572,7 +1570,6 @@ __global__ __launch_bounds__(BLK) void k_merge_hop(const JobConst *__restrict__ 
         m = *y < m ? *y : m;
         *y = m;
       }
-      if (m < INF) atomicOr(&finm[vi][q >> 6], 1ull << (q & 63));
     }
     issued = issue_hop_tile(jcs, lvl, soff, nj, gt + gridDim.x, total, g.n, x);
     __syncthreads();
@@ -1587,30 +1584,27 @@ __global__ __launch_bounds__(BLK) void k_merge_hop(const JobConst *__restrict__ 
     }
     // item order (vertex, height, subset) makes the level-block write below contiguous
     const UDiv dnl(nL), dh(H);
+    const int nsplit = 1 << (L - 1);  // splits per subset of the level, plus one
     for (int it = threadIdx.x; it < tv * nL * H; it += BLK) {
       const int row = (int)dnl.div((uint32_t)it);   // vi * H + nd
       const int j = it - row * nL;
       const int vi = (int)dh.div((uint32_t)row), nd = row - vi * H;
       const int p = lsub[L][j];
-      const unsigned long long f0 = finm[vi][0], f1 = finm[vi][1];
       const int low = p & -p, rest = p ^ low;
       const int b = min(nd, D - nd);
       const int32_t *pn = pm + vi * RW + nd * WS;  // heights <= nd
       const int32_t *pb = pm + vi * RW + b * WS;   // heights <= b
+      // splits p1 + p2 = p with low in p1: sm runs over the proper subsets of rest. The
+      // count is the same for every item of the level, so the loop is uniform; an INF
+      // half gives a sum >= INF (2 INF < 2^31), which never wins
       int m = INF;
-      for (int sm = rest;; sm = (sm - 1) & rest) {
+      int sm = rest;
+      for (int t = 1; t < nsplit; t++) {
+        sm = (sm - 1) & rest;
         const int p1 = low | sm, p2 = p ^ p1;
-        if (p1 != p) {
-          const bool l1 = ((p1 < 64 ? f0 >> p1 : f1 >> (p1 - 64)) & 1) != 0;
-          const bool l2 = ((p2 < 64 ? f0 >> p2 : f1 >> (p2 - 64)) & 1) != 0;
-          if (l1 && l2) {
-            const int t = pn[p1] + pb[p2];
-            const int u = pb[p1] + pn[p2];
-            m = t < m ? t : m;
-            m = u < m ? u : m;
-          }
-        }
-        if (sm == 0) break;
+        const int x = pn[p1] + pb[p2];
+        const int y = pb[p1] + pn[p2];
+        m = min(m, min(x, y));
       }
       if (m >= INF) continue;
       const int q = c.R ^ p;
