@@ -127,3 +127,37 @@ def test_rmat_hop_large_D(gpu_engine, rmat, rmat_inst, D):
         assert r["status"] == engine.GST_OK and r["cost"] == o["tree_cost"] == o["dp_cost"], (q, D, r["cost"], o)
         ok, why = engine.validate_tree(rmat.graph, rmat.groups, q, r["edges"], r["cost"], D)
         assert ok, (q, D, why)
+
+@pytest.mark.parametrize("k", [4, 5, 6, 7])
+def test_rmat_hop_full_tiles(gpu_engine, rmat, rmat_inst, k):
+    """D = 0..2 give the hop merge its widest tiles (244 vertices at k = 4, D = 0, where one
+    staged level segment is 1464 words; 81 at k = 4, D = 2): every level segment must reach
+    LDS. The queries take groups of distinct neighbours of one vertex, so a tree of
+    diameter 2 exists and D = 2 has a finite optimum."""
+    g, gr = rmat.graph, rmat.groups
+    groups_of = {}
+    for gi in range(gr.n_groups):
+        for v in gr.members(gi):
+            groups_of.setdefault(int(v), []).append(gi)
+    rng = np.random.default_rng(1300 + k)
+    n_finite = 0
+    for i in range(2):
+        while True:
+            v = int(rng.integers(g.n))
+            nb = g.adj[g.beg_pos[v]:g.beg_pos[v + 1]]
+            cand = sorted({x for u in nb for x in groups_of.get(int(u), [])})
+            if len(cand) >= k:
+                q = rng.choice(cand, k, replace=False).astype(np.int32)
+                break
+        for D in (0, 1, 2):
+            print("full tiles k", k, "D", D, q, flush=True)
+            r = gpu_engine.solve(q, D=D)
+            o = pyoracle.dprunedpp(rmat_inst, q, D)
+            if o["status"] != 0:
+                assert r["status"] == engine.GST_NO_SOLUTION, (q, D, r)
+                continue
+            n_finite += 1
+            assert r["status"] == engine.GST_OK and r["cost"] == o["tree_cost"] == o["dp_cost"], (q, D, r["cost"], o)
+            ok, why = engine.validate_tree(g, gr, q, r["edges"], r["cost"], D)
+            assert ok, (q, D, why)
+    assert n_finite >= 2
