@@ -502,11 +502,18 @@ __device__ unsigned long long grow_select_hop(const JobConst &c, JobState *s, in
   int dlb[MAXK], dst[MAXK];
 #pragma unroll
   for (int i = 0; i < MAXK; i++) { dlb[i] = INF; dst[i] = INF; }
-  for (int h = 0; h < H; h++) {
+  for (int h0 = 0; h0 < H; h0 += 4) {  // 4 heights of loads in flight at a time
+    int xs[4][MAXK];
 #pragma unroll
-    for (int i = 0; i < MAXK; i++) {
-      if (i < K) {
-        const int x = l1[h * K + i];
+    for (int hh = 0; hh < 4; hh++)
+#pragma unroll
+      for (int i = 0; i < MAXK; i++) xs[hh][i] = (h0 + hh < H && i < K) ? l1[(h0 + hh) * K + i] : INF;
+#pragma unroll
+    for (int hh = 0; hh < 4; hh++) {
+      const int h = h0 + hh;
+#pragma unroll
+      for (int i = 0; i < MAXK; i++) {
+        const int x = xs[hh][i];
         if (h <= D - d && x < dlb[i]) dlb[i] = x;
         if (2 * h <= D && x < dst[i]) dst[i] = x;
       }
@@ -535,10 +542,14 @@ __device__ unsigned long long grow_select_hop(const JobConst &c, JobState *s, in
     if (q && popc(q) < L) {
       const int Lq = popc(q);
       const int32_t *cq = lrow(c, Lq, v, 0) + c.rank[q];
+      const int nq = c.nsub[Lq];
       int mq = INF;
-      for (int d2 = 0; d2 <= D - d; d2++) {
-        const int x = cq[(size_t)d2 * c.nsub[Lq]];
-        mq = x < mq ? x : mq;
+      for (int d0 = 0; d0 <= D - d; d0 += 4) {  // 4 heights of loads in flight at a time
+        int xs[4];
+#pragma unroll
+        for (int dd = 0; dd < 4; dd++) xs[dd] = d0 + dd <= D - d ? cq[(size_t)(d0 + dd) * nq] : INF;
+#pragma unroll
+        for (int dd = 0; dd < 4; dd++) mq = xs[dd] < mq ? xs[dd] : mq;
       }
       if (mq < INF && cst + mq < ubl) { atomicMin(&s->ub, cst + mq); ubl = cst + mq; }
     }
