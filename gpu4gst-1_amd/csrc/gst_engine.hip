@@ -290,9 +290,11 @@ inline unsigned grid_for(long long items, long long per_block, unsigned cap) {
   return (unsigned)b;
 }
 
-// Launch the init kernels of a job (arena = INF, masks = 0) on stream st.
-void launch_job_init(hipStream_t st, const Slot &s, const gst_ctx *c) {
-  k_fill_i32<<<grid_for((long long)s.arena_words / 4, BLK, 8192), BLK, 0, st>>>(s.T, (long long)s.arena_words, INF);
+// Launch the init kernels of a job on stream st: level 1 of the arena = INF (the merge of
+// each level L >= 2 writes every state of its block, INF included), masks = 0.
+void launch_job_init(hipStream_t st, const Slot &s, const gst_ctx *c, const JobConst &jc) {
+  const long long words = std::min<long long>((long long)s.arena_words, jc.off[2]);
+  k_fill_i32<<<grid_for(words / 4, BLK, 8192), BLK, 0, st>>>(s.T, words, INF);
   k_fill_u64<<<grid_for((long long)s.nact * c->n, BLK, 4096), BLK, 0, st>>>(s.act, (long long)s.nact * c->n, 0ull);
   k_fill_u64<<<grid_for(NSTRIPE * NSTX, BLK, 64), BLK, 0, st>>>(s.stx, (long long)NSTRIPE * NSTX, 0ull);
 }
@@ -628,7 +630,7 @@ int gst_solve(gst_ctx *c, const int32_t *gids, int32_t k, int32_t D, int32_t fla
   init_jobstate(js);
   HIPCHK(hipMemcpyAsync(c->d_jc, &jc, sizeof(jc), hipMemcpyHostToDevice, c->st));
   HIPCHK(hipMemcpyAsync(c->d_js, &js, sizeof(js), hipMemcpyHostToDevice, c->st));
-  launch_job_init(c->st, s, c);
+  launch_job_init(c->st, s, c, jc);
   HIPCHK(hipGetLastError());
   const GraphDev g = c->g;
   JobState &h = c->h_js[0];
@@ -719,7 +721,9 @@ int gst_solve(gst_ctx *c, const int32_t *gids, int32_t k, int32_t D, int32_t fla
   finish_result(c, q, fin, edges, out);
   if (q.flags & GST_F_STATS) {
     HIPCHK(hipMemsetAsync(c->d_count, 0, sizeof(unsigned long long), c->st));
-    k_count_finite<<<grid_for((long long)s.arena_words, BLK, 4096), BLK, 0, c->st>>>(s.T, (long long)s.arena_words, c->d_count);
+    // the states of levels 1..k (the arena's last n * H words are unused and not filled)
+    const long long used = std::min<long long>((long long)s.arena_words, jc.off[q.K + 1]);
+    k_count_finite<<<grid_for(used, BLK, 4096), BLK, 0, c->st>>>(s.T, used, c->d_count);
     unsigned long long cnt = 0;
     HIPCHK(hipMemcpyAsync(&cnt, c->d_count, sizeof(cnt), hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));

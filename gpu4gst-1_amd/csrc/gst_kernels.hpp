@@ -1428,9 +1428,11 @@ __global__ __launch_bounds__(BLK) void k_merge(const JobConst *__restrict__ jcs,
         const int p1 = low | sm;
         m = min(m, row[p1] + row[p ^ p1]);
       }
+      // every state of the level is written here, INF included: levels >= 2 are not
+      // filled at query start
+      as_global(c.T)[(size_t)c.off[L] + (size_t)(v0 + vi) * nL + j] = m < INF ? m : INF;
       if (m >= INF) continue;
       merged++;
-      c.T[(size_t)c.off[L] + (size_t)(v0 + vi) * nL + j] = m;
       int sl[MAXK];
 #pragma unroll
       for (int i = 0; i < MAXK; i++) sl[i] = (i < K) ? row[1 << i] : INF;
@@ -1617,21 +1619,26 @@ __global__ __launch_bounds__(BLK) void k_merge_hop(const JobConst *__restrict__ 
         const int y = pb[p1] + pn[p2];
         m = min(m, min(x, y));
       }
-      if (m >= INF) continue;
-      const int q = c.R ^ p;
-      const int32_t *pr = pm + (vi * H + (D - nd)) * WS;  // minima over heights <= D - nd
-      if (q == 0) ubl = m < ubl ? m : ubl;
-      else if (popc(q) < L && (long long)m + pr[q] < ubl) ubl = m + pr[q];
-      if (prune && L < K) {
-        // m + lb > UB: no optimal derivation passes through this state, as a growth
-        // source or as a merge input, so it is not stored at all
-        int lb = 0;
-        for (int i = 0; i < K; i++)
-          if (!((p >> i) & 1)) lb = pr[1 << i] > lb ? pr[1 << i] : lb;
-        if ((long long)m + lb > ub0) continue;
+      bool keep = m < INF;
+      if (keep) {
+        const int q = c.R ^ p;
+        const int32_t *pr = pm + (vi * H + (D - nd)) * WS;  // minima over heights <= D - nd
+        if (q == 0) ubl = m < ubl ? m : ubl;
+        else if (popc(q) < L && (long long)m + pr[q] < ubl) ubl = m + pr[q];
+        if (prune && L < K) {
+          // m + lb > UB: no optimal derivation passes through this state, as a growth
+          // source or as a merge input, so it is not stored (it stays INF)
+          int lb = 0;
+          for (int i = 0; i < K; i++)
+            if (!((p >> i) & 1)) lb = pr[1 << i] > lb ? pr[1 << i] : lb;
+          if ((long long)m + lb > ub0) keep = false;
+        }
       }
+      // every state of the level is written here, INF included: levels >= 2 are not
+      // filled at query start
+      as_global(c.T)[(size_t)c.off[L] + (size_t)v0 * H * nL + it] = keep ? m : INF;
+      if (!keep) continue;
       merged++;
-      as_global(c.T)[(size_t)c.off[L] + (size_t)v0 * H * nL + it] = m;
       if (L < K && nd < D) atomicOr(&amask[vi * H + nd], 1ull << j);
     }
     if (ubl < ub0) atomicMin(&ubs, ubl);

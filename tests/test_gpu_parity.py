@@ -161,3 +161,22 @@ def test_rmat_hop_full_tiles(gpu_engine, rmat, rmat_inst, k):
             ok, why = engine.validate_tree(g, gr, q, r["edges"], r["cost"], D)
             assert ok, (q, D, why)
     assert n_finite >= 2
+
+@pytest.mark.parametrize("k", [2, 3, 4])
+def test_stats_count_unpruned(gpu_engine, rmat, k):
+    """GST_F_STATS without pruning: on a connected graph every state (v, p) is finite, so
+    the count is n (2^k - 1) exactly, for the plain and the batched path. Every state of
+    levels >= 2 must have been written by its level's merge (only level 1 is filled at
+    query start)."""
+    from scipy.sparse import csr_matrix
+    from scipy.sparse.csgraph import connected_components
+    g = rmat.graph
+    a = csr_matrix((np.ones(g.m), g.adj, g.beg_pos), shape=(g.n, g.n))
+    assert connected_components(a, directed=False)[0] == 1
+    rng = np.random.default_rng(1500 + k)
+    q = rng.choice(rmat.groups.n_groups, k, replace=False).astype(np.int32)
+    flags = engine.F_STATS | engine.F_NO_PRUNE
+    r = gpu_engine.solve(q, flags=flags)
+    assert r["status"] == engine.GST_OK and r["states_finite"] == g.n * (2 ** k - 1), (q, r["states_finite"])
+    b = gpu_engine.solve_batch([engine.Query(q, -1, flags)])[0]
+    assert b["cost"] == r["cost"] and b["states_finite"] == g.n * (2 ** k - 1), (q, b["states_finite"])
