@@ -1615,9 +1615,9 @@ __global__ __launch_bounds__(BLK) void k_merge_hop(const JobConst *__restrict__ 
       for (int t = 1; t < nsplit; t++) {
         sm = (sm - 1) & rest;
         const int p1 = low | sm, p2 = p ^ p1;
-        const int x = pn[p1] + pb[p2];
-        const int y = pb[p1] + pn[p2];
-        m = min(m, min(x, y));
+        const int s12 = pn[p1] + pb[p2];
+        const int s21 = pb[p1] + pn[p2];
+        m = min(m, min(s12, s21));
       }
       bool keep = m < INF;
       if (keep) {
