@@ -262,7 +262,7 @@ def main():
     # ---- batched path: its own tuning, warmup, then timed steps
     if not args.tuning and not args.no_tune:
         tuned_b, _ = autotune.tune(eng, tune_q[:min(n_tune, args.batch)],
-                                   grid={"degree_split": (64, 128, 256, 512), "chunk_edges": (1024, 4096, 16384),
+                                   grid={"degree_split": (64, 128, 256, 512), "chunk_edges": (512, 1024, 4096),
                                          "near_far_delta": (0,)}, budget_s=60.0, batched=True)
         print(f"# autotuned (batched) {tuned_b} ({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
     batches = [queries[i * args.batch:(i + 1) * args.batch] for i in range(args.warmup + args.steps)]
