@@ -82,6 +82,25 @@ def test_open_rejects_malformed_input():
         assert _open(**case) == engine.GST_BAD_INPUT, case
 
 
+def test_open_rejects_arc_count_past_limit():
+    """Arc offsets are 32-bit on the device with one chunk of room: m <= 2^31 - 2^25. The
+    size check comes before any array is read, so small arrays stand in for the CSR."""
+    L = engine.lib()
+    beg = np.asarray([0, 1, 2], np.int64)
+    adj = np.asarray([1, 0], np.int32)
+    w = np.asarray([5, 5], np.int32)
+    offs = np.asarray([0, 1], np.int64)
+    verts = np.asarray([0], np.int32)
+    grp = engine.GstGroups(1, offs.ctypes.data, verts.ctypes.data)
+    limit = (1 << 31) - (1 << 25)
+    for m, msg in ((limit + 1, b"bad CSR sizes"), (limit, b"beg_pos must run from 0 to m")):
+        csr = engine.GstCsr(2, m, beg.ctypes.data, adj.ctypes.data, w.ctypes.data)
+        h = ctypes.c_void_p()
+        rc = L.gst_open(ctypes.byref(csr), ctypes.byref(grp), ctypes.byref(engine.GstOpts()), ctypes.byref(h))
+        assert rc == engine.GST_BAD_INPUT, (m, rc)
+        assert L.gst_last_error().startswith(msg), (m, L.gst_last_error())
+
+
 @pytest.mark.skipif(os.path.exists("/dev/kfd") and os.access("/dev/kfd", os.R_OK), reason="a GPU is present")
 def test_open_without_device():
     assert _open([0, 1, 2], [1, 0], [5, 5], [0, 1], [0]) == engine.GST_NO_DEVICE
