@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <iterator>
 #include <array>
 #include <atomic>
 #include <chrono>
@@ -69,8 +70,10 @@ inline void slot_shape(int D, int &nact, int &nlst, int &nblst) {
   else { nact = 3; nlst = 4; nblst = 2; }
 }
 
-constexpr int SCAP = 1 << 16;  // rebuild stack entries
-constexpr int ECAP = 1 << 15;  // tree edges returned
+// Rebuild stack entries: a derivation has k leaves (singleton seeds) and k - 1 merges, so
+// the depth-first stack never holds more than k + 1 entries.
+constexpr int SCAP = 1 << 12;
+constexpr int ECAP = GST_MAX_TREE_EDGES;  // tree edges returned (each grow step adds one)
 
 }  // namespace
 
@@ -84,6 +87,8 @@ struct gst_ctx {
   int32_t *d_nchunk = nullptr;  // per vertex: wavefront chunks (0 = thread item)
   long long *d_goffs = nullptr;
   std::vector<int64_t> h_goffs;
+  std::vector<int32_t> h_gverts;    // group members, caller's ids
+  std::vector<int32_t> comp;        // connected component of each vertex (caller's ids)
   std::vector<int32_t> h_deg;       // degrees in device order
   std::vector<int32_t> old_of_new;  // device vertex -> caller's id (empty: same numbering)
   int32_t n_groups = 0;
@@ -311,6 +316,26 @@ void fold_stats(const unsigned long long *stripes, JobState &js) {
   }
 }
 
+// Some connected component holds a member of every query group (the non-hop
+// feasibility condition; a diameter bound can only make a feasible query infeasible).
+bool comp_feasible(const gst_ctx *c, const Query &q) {
+  std::vector<int32_t> common;
+  for (size_t i = 0; i < q.gids.size(); i++) {
+    std::vector<int32_t> cs;
+    for (int64_t t = c->h_goffs[q.gids[i]]; t < c->h_goffs[q.gids[i] + 1]; t++) cs.push_back(c->comp[c->h_gverts[t]]);
+    std::sort(cs.begin(), cs.end());
+    cs.erase(std::unique(cs.begin(), cs.end()), cs.end());
+    if (i == 0) common.swap(cs);
+    else {
+      std::vector<int32_t> x;
+      std::set_intersection(common.begin(), common.end(), cs.begin(), cs.end(), std::back_inserter(x));
+      common.swap(x);
+    }
+    if (common.empty()) return false;
+  }
+  return true;
+}
+
 void finish_result(const gst_ctx *c, const Query &q, const JobState &js, const std::vector<int32_t> &edges,
                    gst_result *out) {
   std::memset(out, 0, sizeof(*out));
@@ -330,6 +355,20 @@ void finish_result(const gst_ctx *c, const Query &q, const JobState &js, const s
   }
   out->root = -1;
   out->root_height = -1;
+  out->level = js.level;
+  out->ub = -1;
+  out->lb = -1;
+  const int H = q.D >= 0 ? q.D + 1 : 1;
+  auto ext = [&](int32_t v) { return c->old_of_new.empty() ? v : c->old_of_new[v]; };
+  if (js.err == 7 && js.ans != ~0ull) {
+    // the optimum is known; only its tree exceeds the edge buffer
+    const long long idx = (long long)(js.ans & 0xffffffffull);
+    out->status = GST_TREE_OMITTED;
+    out->cost = out->ub = out->lb = (int32_t)(js.ans >> 32);
+    out->root = ext((int32_t)(idx / H));
+    out->root_height = q.D >= 0 ? (int32_t)(idx % H) : -1;
+    return;
+  }
   if (js.err) {
     out->status = GST_INTERNAL;
     out->cost = -1;
@@ -339,15 +378,16 @@ void finish_result(const gst_ctx *c, const Query &q, const JobState &js, const s
     return;
   }
   if (js.ans == ~0ull) {
-    out->status = GST_NO_SOLUTION;
+    // saturated costs can hide a tree only where one exists in the graph at all
+    const bool range = js.sat && comp_feasible(c, q);
+    out->status = range ? GST_COST_RANGE : GST_NO_SOLUTION;
     out->cost = -1;
+    if (range) set_err("costs reached the saturation range and no tree was found");
     return;
   }
-  const int H = q.D >= 0 ? q.D + 1 : 1;
   const long long idx = (long long)(js.ans & 0xffffffffull);
   out->status = GST_OK;
-  out->cost = (int32_t)(js.ans >> 32);
-  auto ext = [&](int32_t v) { return c->old_of_new.empty() ? v : c->old_of_new[v]; };
+  out->cost = out->ub = out->lb = (int32_t)(js.ans >> 32);
   out->root = ext((int32_t)(idx / H));
   out->root_height = q.D >= 0 ? (int32_t)(idx % H) : -1;
   // distinct edges in the caller's ids, sorted
@@ -458,11 +498,18 @@ const char *gst_strerror(int code) {
     case GST_DEVICE_ERROR: return "device error";
     case GST_INTERNAL: return "internal error";
     case GST_NO_DEVICE: return "no HIP device";
+    case GST_TREE_OMITTED: return "optimum found; its tree exceeds GST_MAX_TREE_EDGES and is not returned";
+    case GST_COST_RANGE: return "costs reached the int32 saturation range and no tree was found";
     default: return "unknown status";
   }
 }
 
 int64_t gst_arena_bytes(int32_t n, int32_t k, int32_t D) { return arena_bytes_for(n, k, D); }
+
+int64_t gst_job_bytes(gst_ctx *c, int32_t k, int32_t D) {
+  if (!c || k < 1 || k > MAXK || D > GST_MAX_D) return GST_BAD_INPUT;
+  return (int64_t)slot_bytes(c, k, D < 0 ? -1 : D);
+}
 
 int gst_open(const gst_csr *csr, const gst_groups *groups, const gst_opts *opts, gst_ctx **out) {
   if (!csr || !groups || !out) { set_err("null argument"); return GST_BAD_INPUT; }
@@ -478,10 +525,25 @@ int gst_open(const gst_csr *csr, const gst_groups *groups, const gst_opts *opts,
   if (csr->beg_pos[0] != 0 || csr->beg_pos[n] != m) { set_err("beg_pos must run from 0 to m"); return GST_BAD_INPUT; }
   for (int32_t v = 0; v < n; v++)
     if (csr->beg_pos[v + 1] < csr->beg_pos[v]) { set_err("beg_pos not monotone"); return GST_BAD_INPUT; }
-  for (int64_t e = 0; e < m; e++) {
-    if (csr->adj[e] < 0 || csr->adj[e] >= n) { set_err("adjacency id out of range"); return GST_BAD_INPUT; }
-    if (csr->w[e] < 1 || csr->w[e] > (1 << 20)) { set_err("edge weight outside [1, 2^20]"); return GST_BAD_INPUT; }
+  // arcs checked while the connected components are joined (union-find, path halving)
+  std::vector<int32_t> comp(n);
+  for (int32_t v = 0; v < n; v++) comp[v] = v;
+  auto root = [&](int32_t x) {
+    while (comp[x] != x) { comp[x] = comp[comp[x]]; x = comp[x]; }
+    return x;
+  };
+  for (int32_t v = 0; v < n; v++) {
+    for (int64_t e = csr->beg_pos[v]; e < csr->beg_pos[v + 1]; e++) {
+      const int32_t u = csr->adj[e];
+      if (u < 0 || u >= n) { set_err("adjacency id out of range"); return GST_BAD_INPUT; }
+      if (csr->w[e] < 1 || csr->w[e] > (1 << 20)) { set_err("edge weight outside [1, 2^20]"); return GST_BAD_INPUT; }
+      if (u > v) {
+        const int32_t a = root(v), b = root(u);
+        if (a != b) comp[a < b ? b : a] = a < b ? a : b;
+      }
+    }
   }
+  for (int32_t v = 0; v < n; v++) comp[v] = root(v);
   if (groups->n_groups < 0 || (groups->n_groups > 0 && (!groups->offs || !groups->verts))) {
     set_err("bad group table"); return GST_BAD_INPUT;
   }
@@ -530,6 +592,8 @@ int gst_open(const gst_csr *csr, const gst_groups *groups, const gst_opts *opts,
   c->old_of_new = std::move(dg.old_of_new);
   c->h_goffs.assign(groups->offs, groups->offs + groups->n_groups + 1);
   if (groups->n_groups == 0) c->h_goffs.assign(1, 0);
+  c->h_gverts.assign(groups->verts, groups->verts + total);
+  c->comp = std::move(comp);
   HIPCHK(hipMalloc((void **)&c->d_beg, sizeof(int32_t) * (n + 1)));
   HIPCHK(hipMalloc((void **)&c->d_adj, sizeof(int32_t) * std::max<int64_t>(m, 1)));
   HIPCHK(hipMalloc((void **)&c->d_w, sizeof(int32_t) * std::max<int64_t>(m, 1)));
@@ -612,9 +676,27 @@ int gst_set_tuning(gst_ctx *c, int32_t degree_split, int32_t chunk_edges, int32_
   return GST_OK;
 }
 
+static int solve_one(gst_ctx *c, const int32_t *gids, int32_t k, int32_t D, int32_t flags, gst_result *out);
+
 int gst_solve(gst_ctx *c, const int32_t *gids, int32_t k, int32_t D, int32_t flags, gst_result *out) {
   if (!c || !out) { set_err("null argument"); return GST_BAD_INPUT; }
   std::memset(out, 0, sizeof(*out));
+  // until the result is complete it reads as a device error (every early return below
+  // that is not a query error is one)
+  out->status = GST_DEVICE_ERROR;
+  out->cost = -1;
+  out->ub = out->lb = -1;
+  const int rc = solve_one(c, gids, k, D, flags, out);
+  if (rc < 0 && rc != GST_INTERNAL && rc != GST_COST_RANGE) {
+    gst_result_free(out);
+    out->n_edges = 0;
+    out->status = rc;
+    out->cost = -1;
+  }
+  return rc;
+}
+
+static int solve_one(gst_ctx *c, const int32_t *gids, int32_t k, int32_t D, int32_t flags, gst_result *out) {
   std::lock_guard<std::mutex> lk(c->solve_mu);
   const auto t0 = clk::now();
   Query q;
@@ -710,7 +792,7 @@ int gst_solve(gst_ctx *c, const int32_t *gids, int32_t k, int32_t D, int32_t fla
     dp_round = (ph == PH_MERGE || ph == PH_GROW || ph == PH_HOPCLEAR) && (q.D >= 0 || h.level >= 2);
   }
   std::vector<int32_t> edges;
-  if (h.n_out > 0 && h.ans != ~0ull) {
+  if (h.n_out > 0 && h.ans != ~0ull && !h.err) {
     edges.resize((size_t)3 * std::min(h.n_out, ECAP));
     HIPCHK(hipMemcpy(edges.data(), s.out, sizeof(int32_t) * edges.size(), hipMemcpyDeviceToHost));
   }
@@ -720,7 +802,7 @@ int gst_solve(gst_ctx *c, const int32_t *gids, int32_t k, int32_t D, int32_t fla
   fold_stats(stripes.data(), fin);
   finish_result(c, q, fin, edges, out);
   if (q.flags & GST_F_STATS) {
-    HIPCHK(hipMemsetAsync(c->d_count, 0, sizeof(unsigned long long), c->st));
+    k_fill_u64<<<1, 64, 0, c->st>>>(c->d_count, 1, 0ull);
     // the states of levels 1..k (the arena's last n * H words are unused and not filled)
     const long long used = std::min<long long>((long long)s.arena_words, jc.off[q.K + 1]);
     k_count_finite<<<grid_for(used, BLK, 4096), BLK, 0, c->st>>>(s.T, used, c->d_count);

@@ -109,7 +109,8 @@ struct alignas(16) JobState {
   int32_t far_min;            // min cost deferred to the far pile this round
   int32_t fpar;               // far pile parity (lists 2 + fpar)
   int32_t minc;               // min cost activated by the last merge (level start)
-  int32_t pad2, pad3;
+  int32_t sat;                // a state came within reach of INF (costs may have saturated)
+  int32_t lb;                 // lower bound on the optimum (set after level 1; 0 before)
   unsigned long long st[8];
   unsigned long long stl[8][2];  // per level: edges scanned, state-edge relaxations
 };
@@ -139,6 +140,18 @@ struct GraphDev {
   int32_t chunk;        // edges per wavefront item
   int32_t n_heavy;      // vertices >= n_heavy have deg < split (nchunk 0); n when unknown
 };
+
+// Costs saturate at INF: a grow step c + w or a merge c1 + c2 at or above INF is stored
+// as INF (unreachable). A finite answer is always exact (every state of an optimal
+// derivation costs at most the optimum). A saturated sum needs a finite state of cost
+// >= SAT_AT (a merge half, or a grow source, since weights are <= 2^20 < SAT_AT), and
+// every finite state passes the select pass (when grown) or a merge (when merged), which
+// flag the job then, so that a "no solution" outcome is reported as a cost-range failure.
+constexpr int32_t SAT_AT = INF >> 1;
+__device__ __forceinline__ void flag_saturation(JobState *s) {
+  if (!__hip_atomic_load(&s->sat, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+    __hip_atomic_fetch_or(&s->sat, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 // nchunk[u] without a memory access for light vertices: with vertices numbered by
 // descending degree, those of degree >= split are exactly the ids below n_heavy.
@@ -183,20 +196,27 @@ __device__ __forceinline__ int nxt_list(const JobConst &c, const JobState *s) {
   return c.hop ? s->depth + 1 : (s->parity ^ 1);
 }
 
-// Append vertex u to worklist li: a thread item when deg < split, otherwise one
-// wavefront item per chunk of edges.
-__device__ __forceinline__ void push_vertex(const JobConst &c, JobState *s, const GraphDev &g,
-                                            int li, int32_t u) {
-  const int32_t d = g.beg[u + 1] - g.beg[u];
-  if (d < g.split) {
+// Append vertex u to worklist li: a thread item when nch == 0, otherwise nch wavefront
+// items (one per chunk of edges). The far piles of the near-far frontier take thread
+// items only, whatever the degree, so callers pass nch = 0 for them.
+__device__ __forceinline__ void push_item(const JobConst &c, JobState *s, const GraphDev &g,
+                                          int li, int32_t u, int nch) {
+  if (nch == 0) {
     const int pos = atomicAdd(&s->cnt[li][0], 1);
     c.lst[(size_t)li * g.n + pos] = u;
   } else {
-    const int nch = (d + g.chunk - 1) / g.chunk;
     const int pos = atomicAdd(&s->cnt[li][1], nch);
     int32_t *b = c.blst + ((size_t)li * c.bcap + pos) * 2;
     for (int i = 0; i < nch; i++) { b[2 * i] = u; b[2 * i + 1] = i; }
   }
+}
+
+// Append vertex u to worklist li by its degree class: a thread item when deg < split,
+// otherwise one wavefront item per chunk of edges.
+__device__ __forceinline__ void push_vertex(const JobConst &c, JobState *s, const GraphDev &g,
+                                            int li, int32_t u) {
+  const int32_t d = g.beg[u + 1] - g.beg[u];
+  push_item(c, s, g, li, u, d < g.split ? 0 : (d + g.chunk - 1) / g.chunk);
 }
 
 // Admissible lower bound on the weight still needed to connect a tree rooted at v
@@ -287,6 +307,7 @@ __device__ void step_job(const JobConst &c, JobState &s) {
       goto grow_entry;
     case PH_HOPCLEAR:
       for (int i = 0; i < MAXLISTS; i++) { s.cnt[i][0] = 0; s.cnt[i][1] = 0; }
+      if (s.level == 1) { s.phase = PH_TABLES; return; }  // lower bound from the distances
       s.level++;
       s.depth = 0;
       s.phase = PH_MERGE;
@@ -462,6 +483,7 @@ __device__ __forceinline__ unsigned long long grow_select(const JobConst &c, Job
     const int p = c.sub[L][j];
     const int cst = row[j];
     if (cst >= INF) continue;
+    if (cst >= SAT_AT) flag_saturation(s);
     const int q = c.R ^ p;
     const int compv = (q && popc(q) < L) ? c.T[sidx(c, v, q, 0)] : INF;
     const int cand = ub_candidate(sl, p, cst, L, c, compv);
@@ -538,6 +560,7 @@ __device__ unsigned long long grow_select_hop(const JobConst &c, JobState *s, in
     const int p = c.sub[L][j];
     const int cst = row[j];
     if (cst >= INF) continue;
+    if (cst >= SAT_AT) flag_saturation(s);
     const int q = c.R ^ p;
     if (q && popc(q) < L) {
       const int Lq = popc(q);
@@ -748,9 +771,10 @@ __device__ __forceinline__ void wp_push(WavePush &b, const JobConst *jcs, JobSta
   if (!wm) return;
   const int first = __builtin_ctzll(wm);
   const int jbF = __shfl(jb, first, 64), loF = __shfl(lo, first, 64);
-  if (!__all(!want || (jb == jbF && lo == loF))) {  // mixed jobs in one step: rare
+  if (!__all(!want || (jb == jbF && lo == loF))) {  // mixed jobs or lists in one step: rare
     wp_flush(b, jcs, jss, g);
-    if (want) push_vertex(jcs[jb], &jss[jb], g, lo, u);
+    // by nch_u, as the buffered path does: far-pile pushes are thread items (nch_u = 0)
+    if (want) push_item(jcs[jb], &jss[jb], g, lo, u, nch_u);
     return;
   }
   if (b.jb != jbF || b.lo != loF) {
@@ -1228,6 +1252,15 @@ __global__ __launch_bounds__(BLK) void k_clear(const JobConst *__restrict__ jcs,
 // After level 1: r1 = UB/2 + 1, group distance lower bounds vv[a][b] and Held-Karp tour
 // tables hk[a][b][X] (cheapest walk from group a through every group of X ending at
 // b in X) and w2[a][X] = min_b hk[a][b][X]. One block per job.
+// Also the job's progressive lower bound on the optimum (CPUNONHOP.h:436-496 bounds a
+// state; this bounds the whole query): the Euler tour of an optimal tree visits every
+// query group and costs twice the tree, and consecutive groups on it are joined by
+// walks of at least vv, so OPT >= max_a min_b (hk[a][b][R] + vv[b][a]) / 2; and every
+// pair of groups is joined inside the tree, so OPT >= max vv[a][b]. vv is a valid lower
+// bound on group-to-group distances: values up to UB/2 are exact, and larger ones are
+// clamped to r1 <= their true value. Diameter-bounded jobs take vv over paths of at
+// most D hops (the min over heights of the singletons), which bound the tree's paths
+// between groups in the same way.
 __global__ __launch_bounds__(BLK) void k_tables(const JobConst *__restrict__ jcs, JobState *jss, int njobs, GraphDev g) {
   if ((int)blockIdx.x >= njobs) return;
   const JobConst &c = jcs[blockIdx.x];
@@ -1250,11 +1283,13 @@ __global__ __launch_bounds__(BLK) void k_tables(const JobConst *__restrict__ jcs
     const long long b0 = g.goffs[c.gid[a]], b1 = g.goffs[c.gid[a] + 1];
     for (long long t = b0 + threadIdx.x; t < b1; t += BLK) {
       const int32_t x = g.gverts[t];
-      const int32_t *sing = lrow(c, 1, x, 0);
-      for (int b = 0; b < K; b++) {
-        int d = sing[b];
-        d = d < r1 ? d : r1;
-        atomicMin(&vv[a * MAXK + b], d);
+      for (int h = 0; h < c.H; h++) {  // non-hop: H = 1
+        const int32_t *sing = lrow(c, 1, x, h);
+        for (int b = 0; b < K; b++) {
+          int d = sing[b];
+          d = d < r1 ? d : r1;
+          atomicMin(&vv[a * MAXK + b], d);
+        }
       }
     }
   }
@@ -1278,6 +1313,20 @@ __global__ __launch_bounds__(BLK) void k_tables(const JobConst *__restrict__ jcs
       hk[(a * K + b) * W + X] = val < LB_CAP ? val : LB_CAP;
     }
     __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    int pair = 0, tour = 0;
+    for (int a = 0; a < K; a++) {
+      int mn = INF;
+      for (int b = 0; b < K; b++) {
+        pair = vv[a * MAXK + b] > pair ? vv[a * MAXK + b] : pair;
+        const int t = hk[(a * K + b) * W + c.R] + vv[b * MAXK + a];
+        mn = t < mn ? t : mn;
+      }
+      tour = mn > tour ? mn : tour;
+    }
+    const int lb = (tour + 1) >> 1;
+    s->lb = lb > pair ? lb : pair;
   }
   int32_t *ghk = c.tab, *gw2 = c.tab + K * K * W, *gvv = gw2 + K * W;
   for (int i = threadIdx.x; i < K * K * W; i += BLK) ghk[i] = (i % W) ? hk[i] : 0;
@@ -1432,6 +1481,7 @@ __global__ __launch_bounds__(BLK) void k_merge(const JobConst *__restrict__ jcs,
       // filled at query start
       as_global(c.T)[(size_t)c.off[L] + (size_t)(v0 + vi) * nL + j] = m < INF ? m : INF;
       if (m >= INF) continue;
+      if (m >= SAT_AT) flag_saturation(s);
       merged++;
       int sl[MAXK];
 #pragma unroll
@@ -1557,7 +1607,9 @@ __global__ __launch_bounds__(BLK) void k_merge_hop(const JobConst *__restrict__ 
     __syncthreads();
     const int ub0 = ub0s;
     // rows -> pm[(vi * H + d) * WS + q]; every subset read below has fewer than L members,
-    // so this overwrites all of it
+    // so this overwrites all of it. Height-D singletons are never grown (so the select
+    // pass never sees them): the staged values are checked for saturation here.
+    bool satl = false;
 #pragma unroll
     for (int l = 1; l < MAXK; l++) {
       if (l < L) {
@@ -1568,11 +1620,14 @@ __global__ __launch_bounds__(BLK) void k_merge_hop(const JobConst *__restrict__ 
           const int i = u * BLK + (int)threadIdx.x;
           if (i < n) {
             const int r = (int)dn.div((uint32_t)i);
-            pm[r * WS + lsub[l][i - r * ns]] = x[l - 1][u];
+            const int32_t xv = x[l - 1][u];
+            satl |= xv >= SAT_AT && xv < INF;
+            pm[r * WS + lsub[l][i - r * ns]] = xv;
           }
         }
       }
     }
+    if (satl) flag_saturation(s);
     __syncthreads();
     for (int i = threadIdx.x; i < tv * W; i += BLK) {  // prefix minima over height, in place
       const int vi = i >> c.K, q = i & (W - 1);
@@ -1621,6 +1676,7 @@ __global__ __launch_bounds__(BLK) void k_merge_hop(const JobConst *__restrict__ 
       }
       bool keep = m < INF;
       if (keep) {
+        if (m >= SAT_AT) flag_saturation(s);
         const int q = c.R ^ p;
         const int32_t *pr = pm + (vi * H + (D - nd)) * WS;  // minima over heights <= D - nd
         if (q == 0) ubl = m < ubl ? m : ubl;

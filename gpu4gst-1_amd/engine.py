@@ -19,8 +19,10 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
 
-GST_OK, GST_NO_SOLUTION, GST_PENDING = 0, 1, 2
+GST_OK, GST_NO_SOLUTION, GST_PENDING, GST_TREE_OMITTED = 0, 1, 2, 3
 GST_BAD_INPUT, GST_OVER_BUDGET, GST_DEVICE_ERROR, GST_INTERNAL, GST_NO_DEVICE = -1, -2, -3, -4, -5
+GST_COST_RANGE = -6
+COST_CEILING = 0x3fffffff
 F_STATS, F_NO_TREE, F_NO_PRUNE = 1, 2, 4
 O_KEEP_ORDER = 0x100  # gst_opts.flags: keep the caller's vertex numbering on the device
 MAX_K, MAX_D = 7, 15
@@ -55,7 +57,8 @@ class GstResult(ctypes.Structure):
                 ("states_processed", ctypes.c_int64), ("grow_vertices", ctypes.c_int64),
                 ("grow_edges", ctypes.c_int64), ("grow_relax", ctypes.c_int64), ("grow_pushes", ctypes.c_int64),
                 ("arena_bytes", ctypes.c_int64), ("level_edges", ctypes.c_int64 * 8),
-                ("level_relax", ctypes.c_int64 * 8), ("grow_rounds", ctypes.c_int32)]
+                ("level_relax", ctypes.c_int64 * 8), ("grow_rounds", ctypes.c_int32),
+                ("ub", ctypes.c_int32), ("lb", ctypes.c_int32), ("level", ctypes.c_int32), ("pad_", ctypes.c_int32)]
 
 
 class GstInfo(ctypes.Structure):
@@ -65,7 +68,8 @@ class GstInfo(ctypes.Structure):
 
 
 EXPORTS = ["gst_open", "gst_solve", "gst_solve_batch", "gst_submit", "gst_poll", "gst_wait", "gst_info_get",
-           "gst_set_tuning", "gst_result_free", "gst_close", "gst_strerror", "gst_last_error", "gst_arena_bytes"]
+           "gst_set_tuning", "gst_result_free", "gst_close", "gst_strerror", "gst_last_error", "gst_arena_bytes",
+           "gst_job_bytes"]
 
 
 def lib_path() -> str:
@@ -97,6 +101,8 @@ def lib():
         L.gst_last_error.restype = ctypes.c_char_p
         L.gst_arena_bytes.restype = ctypes.c_int64
         L.gst_arena_bytes.argtypes = [i32, i32, i32]
+        L.gst_job_bytes.restype = ctypes.c_int64
+        L.gst_job_bytes.argtypes = [P, i32, i32]
         _LIB = L
     return _LIB
 
@@ -108,7 +114,7 @@ class GstError(RuntimeError):
 
 
 def _result_dict(r: GstResult) -> dict:
-    d = {f: getattr(r, f) for f, _ in GstResult._fields_ if f not in ("edges", "level_edges", "level_relax")}
+    d = {f: getattr(r, f) for f, _ in GstResult._fields_ if f not in ("edges", "level_edges", "level_relax", "pad_")}
     d["level_edges"] = list(r.level_edges)
     d["level_relax"] = list(r.level_relax)
     n = max(r.n_edges, 0)
@@ -172,10 +178,10 @@ class Engine:
         r = GstResult()
         rc = L.gst_solve(self._h, g.ctypes.data, len(g), int(D), int(flags), ctypes.byref(r))
         try:
-            if rc < 0 and rc != GST_INTERNAL:
+            if rc < 0 and rc not in (GST_INTERNAL, GST_COST_RANGE):
                 raise GstError(rc, L.gst_last_error().decode())
             d = _result_dict(r)
-            if rc == GST_INTERNAL:
+            if rc in (GST_INTERNAL, GST_COST_RANGE):
                 d["error"] = L.gst_last_error().decode()
             return d
         finally:
@@ -232,12 +238,15 @@ class Engine:
         del keep
         return list(tickets)
 
-    def poll(self, ticket):
+    def poll(self, ticket, progress=False):
+        """Result of a finished query; for one in flight None, or with progress=True a dict
+        with status GST_PENDING and the bounds so far (ub: best tree weight found, -1 if
+        none yet; lb: lower bound on the optimum; level: DP level reached)."""
         L = lib()
         r = GstResult()
         rc = L.gst_poll(self._h, ctypes.c_uint64(ticket), ctypes.byref(r))
         if rc == GST_PENDING:
-            return None
+            return _result_dict(r) if progress else None
         if rc < 0 and r.status == 0:
             raise GstError(rc, L.gst_last_error().decode())
         try:
@@ -255,6 +264,13 @@ class Engine:
             return _result_dict(r)
         finally:
             L.gst_result_free(ctypes.byref(r))
+
+    def job_bytes(self, k, D=-1) -> int:
+        """Device bytes one admitted query of this shape is charged against the budget."""
+        b = lib().gst_job_bytes(self._h, int(k), int(D))
+        if b < 0:
+            raise GstError(int(b), "bad query shape")
+        return b
 
     def info(self) -> dict:
         i = GstInfo()

@@ -88,19 +88,25 @@ int main(int argc, char **argv) {
   if (type == 2) {
     for (int i = s; i <= e; i++) {
       rc = gst_solve(ctx, gids + (size_t)i * k, k, D, GST_F_STATS, &res[i - s]);
-      if (rc < 0 && rc != GST_INTERNAL) std::fprintf(stderr, "query %d: %s\n", i, gst_last_error());
+      if (rc < 0) std::fprintf(stderr, "query %d: %s: %s\n", i, gst_strerror(rc), gst_last_error());
     }
   } else {
     std::vector<gst_query> qs;
     for (int i = s; i <= e; i++) qs.push_back({gids + (size_t)i * k, k, D, GST_F_STATS});
     rc = gst_solve_batch(ctx, qs.data(), (int32_t)qs.size(), res.data());
-    if (rc < 0) std::fprintf(stderr, "batch: %s\n", gst_last_error());
+    if (rc < 0) std::fprintf(stderr, "batch: %s: %s\n", gst_strerror(rc), gst_last_error());
   }
+  int failed = 0;
   for (int i = s; i <= e; i++) {
     gst_result &r = res[i - s];
     std::string task;
     for (int j = 0; j < k; j++) task += std::to_string(gids[(size_t)i * k + j]) + " ";
-    const int cost = r.status == GST_OK ? r.cost : none;
+    // errors are not "no solution": their rows carry cost -1 and the exit status is 1
+    if (r.status < 0) {
+      failed++;
+      std::fprintf(stderr, "query %d: %s\n", i, gst_strerror(r.status));
+    }
+    const int cost = (r.status == GST_OK || r.status == GST_TREE_OMITTED) ? r.cost : r.status < 0 ? -1 : none;
     const long long mem = (long long)r.states_finite + (long long)gstd_num_vertices(g) * (1ll << r.k) * (hop ? D + 1 : 1);
     std::fprintf(f, "%d,%s,%.8f,%d,%lld,%lld,%lld\n", i, task.c_str(), r.t_dp_s, cost, mem,
                  (long long)r.states_finite, (long long)r.states_processed);
@@ -114,5 +120,5 @@ int main(int argc, char **argv) {
   gstd_free_buf(goffs);
   gstd_free_buf(gverts);
   gstd_free_buf(gids);
-  return 0;
+  return failed ? 1 : 0;
 }

@@ -23,8 +23,10 @@
  *               queries of mixed k and D share the device, device-side phase control,
  *               the round loop captured in a HIP graph.
  *   gst_submit / gst_poll / gst_wait  serving: queries are admitted as batch slots and
- *               DP memory (budget V*2^k*(D+1)*4 B per query) free up; results are
- *               identical to gst_solve's.
+ *               DP memory (V*2^k*(D+1)*4 B of arena per query, plus its masks and
+ *               worklists: gst_job_bytes) free up; results are identical to gst_solve's.
+ *               gst_poll on a query in flight reports its upper and lower bounds
+ *               (progressive reporting, code/PrunedDP++/include/CPUNONHOP.h:684-726).
  *
  * Conventions (reference): group ids are 0-based indices into the group table (the .g
  * line "g<n>" is group n-1, read_for_GSTP.h:131-133); duplicate ids in a query are
@@ -45,16 +47,26 @@ extern "C" {
 enum {
   GST_OK = 0,
   GST_NO_SOLUTION = 1,  /* some query group cannot be connected to the others */
-  GST_PENDING = 2,      /* gst_poll: not finished yet */
+  GST_PENDING = 2,      /* gst_poll: not finished yet (the result holds the progress so far) */
+  GST_TREE_OMITTED = 3, /* cost is the optimum; the tree exceeds GST_MAX_TREE_EDGES and is
+                           not returned */
   GST_BAD_INPUT = -1,   /* malformed graph/groups/query, k > GST_MAX_K, D > GST_MAX_D */
-  GST_OVER_BUDGET = -2, /* the query's DP arena does not fit the memory budget */
+  GST_OVER_BUDGET = -2, /* the query's DP memory does not fit the memory budget */
   GST_DEVICE_ERROR = -3,
   GST_INTERNAL = -4,    /* reconstruction failed (never expected) */
-  GST_NO_DEVICE = -5
+  GST_NO_DEVICE = -5,
+  GST_COST_RANGE = -6   /* no tree below GST_COST_CEILING was found and some cost reached
+                           the saturation range: the optimum, if any, is not representable */
 };
 
 #define GST_MAX_K 7
 #define GST_MAX_D 15
+/* DP costs are int32 and saturate at 2^30 - 1; every optimum below GST_COST_CEILING is
+ * exact. A query whose costs reach 2^29 and that finds no tree returns GST_COST_RANGE
+ * rather than GST_NO_SOLUTION. */
+#define GST_COST_CEILING 0x3fffffff
+/* Trees of more edges than this are not returned (status GST_TREE_OMITTED). */
+#define GST_MAX_TREE_EDGES (1 << 20)
 /* Arc offsets are 32-bit on the device, with room for one chunk past the end. */
 #define GST_MAX_ARCS ((int64_t)2147483648 - 33554432)
 
@@ -124,6 +136,15 @@ typedef struct {
   int64_t level_edges[8]; /* grow work per DP level (subset size 1..k): edges scanned */
   int64_t level_relax[8]; /*   and state-edge relaxations */
   int32_t grow_rounds;    /* grow rounds executed (one select + grow launch set each) */
+  /* Progress (the anytime view of PrunedDP++, code/PrunedDP++/include/CPUNONHOP.h:3-5,
+   * 684-726). gst_poll returns GST_PENDING with these filled for a query in flight;
+   * a final result has ub == lb == cost when status is GST_OK. */
+  int32_t ub;             /* weight of the best feasible tree found so far (-1: none yet);
+                             never increases between polls and is >= the optimum */
+  int32_t lb;             /* lower bound on the optimum (0 until the group distances are
+                             final); never decreases and is <= the optimum */
+  int32_t level;          /* DP level (subset size) the query has reached */
+  int32_t pad_;
 } gst_result;
 
 typedef struct {
@@ -155,6 +176,10 @@ void gst_close(gst_ctx *ctx);
 const char *gst_strerror(int code);
 const char *gst_last_error(void);
 int64_t gst_arena_bytes(int32_t n, int32_t k, int32_t D);
+/* Device memory one admitted query of this shape holds in the batched and served forms
+ * (arena, activation masks, worklists, tables, tree buffers): what admission charges
+ * against dp_budget_bytes. < 0 on bad arguments. */
+int64_t gst_job_bytes(gst_ctx *ctx, int32_t k, int32_t D);
 
 #ifdef __cplusplus
 }

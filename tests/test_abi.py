@@ -51,7 +51,7 @@ def test_arena_bytes():
 
 def test_strerror():
     L = engine.lib()
-    for code in (0, 1, 2, -1, -2, -3, -4, -5):
+    for code in (0, 1, 2, 3, -1, -2, -3, -4, -5, -6):
         assert L.gst_strerror(code)
 
 

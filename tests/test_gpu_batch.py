@@ -68,9 +68,11 @@ def test_batch_small_budget_serialises(rmat, rmat_inst):
     # a budget that fits the largest arena of the set, but never all of them at once,
     # forces queuing; results unchanged
     qs = _mixed_queries(rmat, 10, 2)
-    arenas = [engine.arena_bytes(rmat.graph.n, len(q.gids), q.D) for q in qs]
-    budget = max(arenas) + 1
-    assert sum(arenas) > budget
+    with engine.Engine(rmat.graph, rmat.groups, max_batch=8) as probe:
+        charged = [probe.job_bytes(len(q.gids), q.D) for q in qs]
+    assert all(c > engine.arena_bytes(rmat.graph.n, len(q.gids), q.D) for c, q in zip(charged, qs))
+    budget = max(charged) + 1
+    assert sum(charged) > budget
     with engine.Engine(rmat.graph, rmat.groups, dp_budget_bytes=budget, max_batch=8) as eng:
         res = eng.solve_batch(qs)
         for q, r in zip(qs, res):
@@ -78,6 +80,7 @@ def test_batch_small_budget_serialises(rmat, rmat_inst):
             _assert_same(rmat, rmat_inst, q, r, alone)
         info = eng.info()
         assert info["completed"] >= 10 and info["in_use_bytes"] == 0
+        assert info["budget_bytes"] == budget
 
 
 def test_serving_continuous_admission(gpu_engine, rmat, rmat_inst):
@@ -101,6 +104,60 @@ def test_serving_continuous_admission(gpu_engine, rmat, rmat_inst):
     for q, t in zip(qs, tickets):
         alone = gpu_engine.solve(q.gids, D=q.D)
         _assert_same(rmat, rmat_inst, q, got[t], alone)
+
+
+@pytest.mark.parametrize("delta", [8, 64])
+def test_near_far_plain_and_mixed_batch(rmat, rmat_inst, delta):
+    """Near-far frontier (states above the threshold wait in a far pile) on the plain path
+    and in batches that mix jobs, whose wavefronts then push for several jobs and lists
+    in one step: every result equals the oracle and the label-correcting default."""
+    qs = _mixed_queries(rmat, 24, 40 + delta)
+    with engine.Engine(rmat.graph, rmat.groups, device=0, near_far_delta=delta) as eng:
+        for split, chunk in [(32, 1024), (4, 64)]:  # (4, 64): hubs as many wavefront items
+            eng.set_tuning(split, chunk, delta)
+            print("near-far delta", delta, "tuning", split, chunk, flush=True)
+            res = eng.solve_batch(qs)
+            for q, r in zip(qs, res):
+                alone = eng.solve(q.gids, D=q.D)
+                _assert_same(rmat, rmat_inst, q, r, alone)
+
+
+def test_progressive_bounds(rmat, rmat_inst):
+    """gst_poll on queries in flight reports the best tree weight found so far (ub) and a
+    lower bound (lb): ub never increases, lb never decreases, lb <= optimum <= ub at every
+    poll, and the final result has ub == lb == cost (the anytime view of PrunedDP++,
+    CPUNONHOP.h:684-726)."""
+    rng = np.random.default_rng(77)
+    qs = [engine.Query(rng.choice(rmat.groups.n_groups, 7, replace=False).astype(np.int32), D)
+          for D in (-1, -1, -1, 4)]
+    with engine.Engine(rmat.graph, rmat.groups, device=0, graph_rounds=1) as eng:
+        tickets = eng.submit(qs)
+        seen = {t: [] for t in tickets}
+        final = {}
+        deadline = time.time() + 60
+        while len(final) < len(tickets) and time.time() < deadline:
+            for t in tickets:
+                if t in final:
+                    continue
+                r = eng.poll(t, progress=True)
+                if r["status"] == engine.GST_PENDING:
+                    seen[t].append((r["ub"], r["lb"], r["level"]))
+                else:
+                    final[t] = r
+        assert len(final) == len(tickets)
+        n_interim = 0
+        for q, t in zip(qs, tickets):
+            r = final[t]
+            oc = _oracle_cost(rmat_inst, q)
+            assert r["status"] == engine.GST_OK and r["cost"] == oc, (q, r["cost"], oc)
+            assert r["ub"] == r["lb"] == r["cost"]
+            ubs = [u for u, _, _ in seen[t] if u >= 0]
+            lbs = [lo for _, lo, _ in seen[t]]
+            assert all(a >= b for a, b in zip(ubs, ubs[1:])), ubs
+            assert all(a <= b for a, b in zip(lbs, lbs[1:])), lbs
+            assert all(u >= oc for u in ubs) and all(lo <= oc for lo in lbs), (oc, ubs, lbs)
+            n_interim += sum(1 for u, lo, _ in seen[t] if u >= 0 and lo > 0)
+        assert n_interim > 0  # some poll saw both bounds before the end
 
 
 def test_serving_rejects_bad_and_oversized(rmat):
