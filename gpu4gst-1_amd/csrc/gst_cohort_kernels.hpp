@@ -1,0 +1,280 @@
+// gst_cohort_kernels.hpp — device code of cohorts: up to 64 queries of one k whose grow
+// phases run together (gfx950, wave64).
+//
+// A cohort's grow rounds relax one level block in which the row of vertex v holds the
+// states of every member at that level, member after member:
+//   CT[v * S + q * C + j] = T_q(v, j-th subset of size L),  C = C(k, L), S = cohort_stride.
+// A relaxation of arc (v, u) then reads u's whole row with coalesced loads, 64 labels per
+// wavefront load, instead of one random cache line per member: on a power-law graph the
+// members' frontiers overlap almost completely (most vertices are active in the same
+// rounds for most labels), so one adjacency scan and one row fetch serve all of them.
+// The merges, tour tables, final minimum and tree rebuild stay per member (the per-query
+// kernels, on "passive" jobs whose phases the host sets): a member's merge writes its
+// level into the cohort block directly, and the block is copied back into the members'
+// own level-major arenas after the level's grow rounds (k_cscatter), where the later
+// merges, the lower bounds and the rebuild read it.
+#pragma once
+#include "gst_kernels.hpp"
+
+namespace gst {
+
+constexpr int CMAXB = 64;   // members per cohort
+constexpr int CMAXCH = 16;  // 64-label chunks per cohort row (row width <= 1024 labels)
+
+struct CohortConst {
+  int32_t *CT;               // [n][S] states of the current level
+  unsigned long long *cact;  // [2][n][NW] label masks per list (bit l = label l of the row)
+  int32_t *flag;             // [2][n] vertex activated in this list (next frontier)
+  int32_t *items;            // [2][icap][2] (vertex, chunk) frontier items; chunk -1 = all arcs
+  unsigned long long *stx;   // [NSTRIPE][NSTX] cohort work counters
+  long long icap;
+  int32_t B, K, L, C, S, NW; // members, k, level, labels per member, row words, chunks
+  int32_t jobs[CMAXB];       // member index -> job index in the cohort's job table
+};
+
+struct CohortState {
+  int32_t par;     // list consumed this round
+  int32_t cnt[2];  // items per list
+  int32_t pad;
+};
+
+// Bits [l0, l0 + C) of a row's label masks (C <= 64).
+__device__ __forceinline__ unsigned long long mask_bits(const unsigned long long *row, int l0, int C) {
+  const int w0 = l0 >> 6, sh = l0 & 63;
+  unsigned long long a = row[w0] >> sh;
+  if (sh + C > 64) a |= row[w0 + 1] << (64 - sh);
+  return C == 64 ? a : a & ((1ull << C) - 1);
+}
+
+// Seeds: T_q(x, {i}) = 0 for every member x of member q's group i (one block per member).
+__global__ __launch_bounds__(BLK) void k_cseed(const CohortConst *__restrict__ ccp, const JobConst *__restrict__ jcs,
+                                               GraphDev g) {
+  const CohortConst &cc = *ccp;
+  const int q = blockIdx.x;
+  if (q >= cc.B) return;
+  const JobConst &c = jcs[cc.jobs[q]];
+  for (int i = 0; i < c.K; i++) {
+    const long long b0 = g.goffs[c.gid[i]], b1 = g.goffs[c.gid[i] + 1];
+    const int l = q * cc.C + i;
+    for (long long t = b0 + threadIdx.x; t < b1; t += BLK) {
+      const int32_t x = g.gverts[t];
+      cc.CT[(size_t)x * cc.S + l] = 0;
+      atomicOr(&cc.cact[((size_t)1 * g.n + x) * cc.NW + (l >> 6)], 1ull << (l & 63));
+      cc.flag[(size_t)1 * g.n + x] = 1;
+    }
+  }
+}
+
+// Frontier of list nx: every flagged vertex becomes one item (or one per chunk of arcs
+// for vertices of degree >= split); the flag is cleared. Wavefront-aggregated appends.
+__global__ __launch_bounds__(BLK) void k_ccollect(const CohortConst *__restrict__ ccp, CohortState *cs, GraphDev g,
+                                                  int nx) {
+  const CohortConst &cc = *ccp;
+  gi32 *flag = as_global(cc.flag + (size_t)nx * g.n);
+  int32_t *items = cc.items + (size_t)nx * cc.icap * 2;
+  const long long stride = (long long)gridDim.x * BLK;
+  for (long long b = (long long)blockIdx.x * BLK + (threadIdx.x & ~63); b < g.n; b += stride) {
+    const long long v = b + lane_id();
+    int cnt = 0, nch = 0;
+    if (v < g.n && flag[v]) {
+      flag[v] = 0;
+      nch = chunks_of(g, (int32_t)v);
+      cnt = nch ? nch : 1;
+    }
+    const int pos = wave_append(&cs->cnt[nx], cnt);
+    if (cnt) {
+      gi32 *it = as_global(items) + (size_t)pos * 2;
+      if (!nch) { it[0] = (int32_t)v; it[1] = -1; }
+      for (int i = 0; i < nch; i++) { it[2 * i] = (int32_t)v; it[2 * i + 1] = i; }
+    }
+  }
+}
+
+// Select pass: a wavefront per frontier vertex, lane q = member q, which evaluates its
+// active states at v as the per-query select pass does (UB candidates, UB/2 and lower
+// bound cuts) and drops the bits of the states that do not grow.
+__global__ __launch_bounds__(BLK) void k_cselect(const CohortConst *__restrict__ ccp, const CohortState *__restrict__ csp,
+                                                 const JobConst *jcs, JobState *jss, GraphDev g) {
+  const CohortConst &cc = *ccp;
+  const int cur = csp->par;
+  const long long total = csp->cnt[cur];
+  const int lane = lane_id();
+  const long long waves = (long long)gridDim.x * (BLK / WAVE);
+  const int L = cc.L, C = cc.C;
+  unsigned long long n_states = 0;
+  for (long long it = (long long)blockIdx.x * (BLK / WAVE) + (threadIdx.x >> 6); it < total; it += waves) {
+    const int32_t *item = cc.items + ((size_t)cur * cc.icap + it) * 2;
+    if (item[1] > 0) continue;  // one evaluation per vertex
+    const int32_t v = item[0];
+    if (lane >= cc.B) continue;
+    unsigned long long *row_m = cc.cact + ((size_t)cur * g.n + v) * cc.NW;
+    const int l0 = lane * C;
+    const unsigned long long a = mask_bits(row_m, l0, C);
+    if (!a) continue;
+    const JobConst &c = jcs[cc.jobs[lane]];
+    JobState *s = &jss[cc.jobs[lane]];
+    const int32_t *row = cc.CT + (size_t)v * cc.S + l0;
+    const int32_t *sing = L == 1 ? row : lrow(c, 1, v, 0);
+    int ns = 0, dmin = INF;
+    unsigned long long defer = 0;
+    const unsigned long long gm = grow_select_rows(c, s, v, row, sing, a, L, load_ub(s), ns, INF, defer, dmin);
+    n_states += ns;
+    const unsigned long long drop = a & ~gm;
+    if (drop) {
+      const int w0 = l0 >> 6, sh = l0 & 63;
+      atomicAnd(&row_m[w0], ~(drop << sh));
+      if (sh + C > 64) atomicAnd(&row_m[w0 + 1], ~(drop >> (64 - sh)));
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) n_states += __shfl_xor(n_states, off, 64);
+  if (lane == 0 && n_states) atomicAdd(cc.stx + (blockIdx.x & (NSTRIPE - 1)) * NSTX + ST_STATES, n_states);
+}
+
+// Grow: a wavefront per item (vertex, chunk of arcs); lanes over the row's labels, 64 per
+// chunk. v's values of its active labels stay in registers; for each arc (u, w), the
+// target row's active chunks are read with coalesced loads, CG_AU arcs in flight at a
+// time; an improvement is an atomicMin (fire-and-forget), the improved bits are OR-ed
+// into u's masks of the next list by one lane per chunk, and u is flagged for the next
+// frontier (k_ccollect builds it).
+constexpr int CG_AU = 4;
+__global__ __launch_bounds__(BLK) void k_cgrow(const CohortConst *__restrict__ ccp, const CohortState *__restrict__ csp,
+                                               GraphDev g) {
+  const CohortConst &cc = *ccp;
+  const int cur = csp->par, nx = cur ^ 1;
+  const long long total = csp->cnt[cur];
+  const int lane = lane_id();
+  const int S = cc.S, NW = cc.NW;
+  const long long waves = (long long)gridDim.x * (BLK / WAVE);
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  gi32 *CT = as_global(cc.CT);
+  const gu64 *actc = as_global(cc.cact + (size_t)cur * g.n * NW);
+  gu64 *actn = as_global(cc.cact + (size_t)nx * g.n * NW);
+  gi32 *flagn = as_global(cc.flag + (size_t)nx * g.n);
+  const gi32 *itm = as_global(cc.items + (size_t)cur * cc.icap * 2);
+  const int32_t *__restrict__ adj = g.adj;
+  const int32_t *__restrict__ wts = g.w;
+  unsigned long long n_edges = 0, n_relax = 0;
+  for (long long it = (long long)blockIdx.x * (BLK / WAVE) + wave; it < total; it += waves) {
+    const int32_t v = __builtin_amdgcn_readfirstlane(itm[2 * it]);
+    const int32_t ch = __builtin_amdgcn_readfirstlane(itm[2 * it + 1]);
+    unsigned long long m[CMAXCH];
+    int32_t sv[CMAXCH];
+    int nb = 0;
+#pragma unroll
+    for (int c = 0; c < CMAXCH; c++) {
+      unsigned long long x = c < NW ? actc[(size_t)v * NW + c] : 0ull;
+      x = ((unsigned long long)(uint32_t)__builtin_amdgcn_readfirstlane((int)(x >> 32)) << 32) |
+          (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)x);
+      m[c] = x;
+      nb += __popcll(x);
+      sv[c] = ((x >> lane) & 1) ? CT[(size_t)v * S + c * 64 + lane] : INF;
+    }
+    if (!nb) continue;
+    const int32_t vb = g.beg[v], ve = g.beg[v + 1];
+    const int32_t eb = ch < 0 ? vb : vb + ch * g.chunk;
+    const int32_t ee = ch < 0 ? ve : min(ve, eb + g.chunk);
+    n_edges += ee - eb;
+    n_relax += (unsigned long long)(ee - eb) * nb;
+    for (int32_t e0 = eb; e0 < ee; e0 += WAVE) {
+      const int na = min(WAVE, ee - e0);
+      const int32_t au = lane < na ? adj[e0 + lane] : 0;
+      const int32_t aw = lane < na ? wts[e0 + lane] : 0;
+      for (int j = 0; j < na; j += CG_AU) {  // uniform trip count
+        int32_t u[CG_AU], wt[CG_AU];
+        bool ok[CG_AU];
+#pragma unroll
+        for (int a = 0; a < CG_AU; a++) {
+          ok[a] = j + a < na;
+          u[a] = __builtin_amdgcn_readlane(au, min(j + a, WAVE - 1));
+          wt[a] = __builtin_amdgcn_readlane(aw, min(j + a, WAVE - 1));
+        }
+        int32_t t[CG_AU][CMAXCH];
+#pragma unroll
+        for (int a = 0; a < CG_AU; a++)
+#pragma unroll
+          for (int c = 0; c < CMAXCH; c++)
+            t[a][c] = (ok[a] && ((m[c] >> lane) & 1)) ? CT[(size_t)u[a] * S + c * 64 + lane] : 0;
+#pragma unroll
+        for (int a = 0; a < CG_AU; a++) {
+          if (!ok[a]) continue;
+          unsigned long long imc = 0;  // lane c: improved bits of chunk c
+          bool any = false;
+#pragma unroll
+          for (int c = 0; c < CMAXCH; c++) {
+            if (!m[c]) continue;  // uniform
+            int nc = sv[c] + wt[a];
+            nc = nc < INF ? nc : INF;
+            const bool imp = ((m[c] >> lane) & 1) && nc < t[a][c];
+            if (imp)
+              __hip_atomic_fetch_min(&CT[(size_t)u[a] * S + c * 64 + lane], nc, __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_AGENT);
+            const unsigned long long im = __ballot(imp);
+            if (lane == c) imc = im;
+            any = any || im != 0;
+          }
+          if (any) {
+            if (imc)
+              __hip_atomic_fetch_or(&actn[(size_t)u[a] * NW + lane], imc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (lane == 0) flagn[u[a]] = 1;
+          }
+        }
+      }
+    }
+  }
+  if (lane == 0 && n_edges) {
+    unsigned long long *st = cc.stx + (blockIdx.x & (NSTRIPE - 1)) * NSTX;
+    atomicAdd(st + ST_EDGES, n_edges);
+    atomicAdd(st + ST_RELAX, n_relax);
+    atomicAdd(st + SX_LEVEL + 2 * (cc.L & 7), n_edges);
+    atomicAdd(st + SX_LEVEL + 2 * (cc.L & 7) + 1, n_relax);
+  }
+}
+
+// Zero the label masks of the list just consumed (every item of a vertex read them).
+__global__ __launch_bounds__(BLK) void k_cclear(const CohortConst *__restrict__ ccp, const CohortState *__restrict__ csp,
+                                                GraphDev g) {
+  const CohortConst &cc = *ccp;
+  const int cur = csp->par;
+  const long long total = (long long)csp->cnt[cur] * cc.NW;
+  for (long long i = (long long)blockIdx.x * BLK + threadIdx.x; i < total; i += (long long)gridDim.x * BLK) {
+    const long long it = i / cc.NW;
+    const int c = (int)(i - it * cc.NW);
+    const int32_t *item = cc.items + ((size_t)cur * cc.icap + it) * 2;
+    if (item[1] > 0) continue;  // the vertex's first item clears its row
+    cc.cact[((size_t)cur * g.n + item[0]) * cc.NW + c] = 0;
+  }
+}
+
+// Copy the level block back into the members' level-major arenas (a block per row range,
+// each thread a fixed set of labels of the row).
+__global__ __launch_bounds__(BLK) void k_cscatter(const CohortConst *__restrict__ ccp, const JobConst *__restrict__ jcs,
+                                                  GraphDev g) {
+  const CohortConst &cc = *ccp;
+  const int L = cc.L, C = cc.C, S = cc.S, nl = cc.B * C;
+  for (int l = threadIdx.x; l < nl; l += BLK) {
+    const int q = l / C, j = l - q * C;
+    const JobConst &c = jcs[cc.jobs[q]];
+    gi32 *dst = as_global(c.T + c.off[L] + j);
+    const gi32 *src = as_global(cc.CT + l);
+    for (long long v = blockIdx.x; v < g.n; v += gridDim.x) dst[(size_t)v * C] = src[(size_t)v * S];
+  }
+}
+
+// Phases of the member jobs, set by the cohort driver (the per-query kernels of that phase
+// then run on them). mode 1: after the final minimum, REBUILD where a tree is wanted,
+// DONE otherwise.
+__global__ void k_cset(JobState *jss, const int32_t *jobs, int nj, int phase, int level, int parity, int mode,
+                       const JobConst *jcs) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nj) return;
+  JobState &s = jss[jobs[i]];
+  if (mode == 1) {
+    s.phase = (s.ans != ~0ull && !(jcs[jobs[i]].flags & JF_NO_TREE)) ? PH_REBUILD : PH_DONE;
+    return;
+  }
+  s.phase = phase;
+  s.level = level;
+  s.parity = parity;
+}
+
+}  // namespace gst

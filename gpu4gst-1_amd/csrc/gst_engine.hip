@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <cstddef>
 #include <deque>
 #include <map>
 #include <list>
@@ -29,20 +30,23 @@
 
 #include "../../include/gst.h"
 #include "gst_kernels.hpp"
+#include "gst_cohort_kernels.hpp"
 
 using namespace gst;
 using clk = std::chrono::steady_clock;
 
 namespace {
 
-thread_local std::string g_err;
-void set_err(const std::string &s) { g_err = s; }
+// Last error of the calling thread (a fixed buffer: setting it never allocates).
+thread_local char g_err[512];
+void set_err(const char *s) { std::snprintf(g_err, sizeof g_err, "%s", s); }
+void set_err(const std::string &s) { set_err(s.c_str()); }
 
 #define HIPCHK(x)                                                                    \
   do {                                                                               \
     hipError_t e_ = (x);                                                             \
     if (e_ != hipSuccess) {                                                          \
-      set_err(std::string(#x) + ": " + hipGetErrorString(e_));                      \
+      std::snprintf(g_err, sizeof g_err, "%s: %s", #x, hipGetErrorString(e_));      \
       return GST_DEVICE_ERROR;                                                       \
     }                                                                                \
   } while (0)
@@ -61,6 +65,7 @@ struct Slot {
   unsigned long long *stx = nullptr;  // striped work counters
   size_t bytes = 0;
   int nact = 0, nlst = 0, nblst = 0;  // mask arrays, thread lists, wavefront lists
+  int ecap = 0;                       // tree edges the out buffer holds
 };
 
 // Non-hop: near masks 0/1 + far-pile mask 2; near lists 0/1 + far piles 2/3 (thread
@@ -105,6 +110,9 @@ struct gst_ctx {
   JobConst *d_jc = nullptr;  // [max_batch]
   JobState *d_js = nullptr;  // [max_batch]
   JobState *h_js = nullptr;  // pinned mirror [max_batch]
+  JobConst *h_jc = nullptr;  // pinned staging of the single-query path's job constants
+  unsigned long long *h_stx = nullptr;    // pinned [NSTRIPE * NSTX] counter stripes
+  unsigned long long *h_count = nullptr;  // pinned finite-state count
   Plan *d_plan = nullptr;    // work layout of the single-query path
   unsigned long long *d_count = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;  // bracket the work kernels of one round
@@ -123,19 +131,33 @@ int64_t arena_bytes_for(int64_t n, int K, int D) {
   return n * ((int64_t)1 << K) * H * 4;
 }
 
+// Lays out the buffers of a slot for (K, D) from base, in 256-byte aligned pieces, with
+// room for ecap tree edges; returns the bytes used. With base == nullptr it only counts.
+size_t slot_carve(const gst_ctx *c, Slot &s, char *base, int K, int D, int ecap) {
+  s.K = K; s.D = D;
+  s.H = D >= 0 ? D + 1 : 1;
+  s.W = 1 << K;
+  s.ecap = ecap;
+  slot_shape(D, s.nact, s.nlst, s.nblst);
+  s.arena_words = (size_t)c->n * s.W * s.H;
+  size_t off = 0;
+  auto take = [&](size_t bytes) { char *p = base ? base + off : nullptr; off += (bytes + 255) & ~(size_t)255; return p; };
+  s.T = (int32_t *)take(s.arena_words * 4);
+  s.act = (unsigned long long *)take((size_t)s.nact * c->n * 8);
+  s.lst = (int32_t *)take((size_t)s.nlst * c->n * 4);
+  s.blst = (int32_t *)take((size_t)s.nblst * c->bcap * 8);
+  s.tab = (int32_t *)take((size_t)(MAXK * MAXK * MAXW + MAXK * MAXW + MAXK * MAXK) * 4);
+  s.stk = (int32_t *)take((size_t)SCAP * 12);
+  s.out = (int32_t *)take((size_t)ecap * 12);
+  s.stx = (unsigned long long *)take((size_t)NSTRIPE * NSTX * 8);
+  s.bytes = off;
+  return off;
+}
+
 // bytes of a slot for (K, D)
-size_t slot_bytes(const gst_ctx *c, int K, int D) {
-  const int H = D >= 0 ? D + 1 : 1, W = 1 << K;
-  int nact, nlst, nblst;
-  slot_shape(D, nact, nlst, nblst);
-  size_t b = (size_t)c->n * W * H * 4;                 // arena
-  b += (size_t)nact * c->n * 8;                         // masks
-  b += (size_t)nlst * c->n * 4;                         // thread lists
-  b += (size_t)nblst * c->bcap * 8;                     // wavefront lists
-  b += (size_t)(MAXK * MAXK * MAXW + MAXK * MAXW + MAXK * MAXK) * 4;  // tables
-  b += (size_t)SCAP * 12 + (size_t)ECAP * 12;
-  b += (size_t)NSTRIPE * NSTX * 8;
-  return b;
+size_t slot_bytes(const gst_ctx *c, int K, int D, int ecap = ECAP) {
+  Slot s;
+  return slot_carve(c, s, nullptr, K, D, ecap);
 }
 
 void slot_free(Slot &s) {
@@ -145,45 +167,16 @@ void slot_free(Slot &s) {
 
 int slot_alloc(gst_ctx *c, Slot &s, int K, int D) {
   slot_free(s);
-  s.K = K; s.D = D;
-  s.H = D >= 0 ? D + 1 : 1;
-  s.W = 1 << K;
-  slot_shape(D, s.nact, s.nlst, s.nblst);
-  s.arena_words = (size_t)c->n * s.W * s.H;
-  s.bytes = slot_bytes(c, K, D);
   char *base = nullptr;
-  hipError_t e = hipMalloc((void **)&base, s.bytes);
+  const size_t bytes = slot_bytes(c, K, D);
+  hipError_t e = hipMalloc((void **)&base, bytes);
   if (e != hipSuccess) {
     (void)hipGetLastError();
     set_err(std::string("hipMalloc(slot) failed: ") + hipGetErrorString(e));
     s = Slot();
     return GST_OVER_BUDGET;
   }
-  size_t off = 0;
-  auto take = [&](size_t bytes) { char *p = base + off; off += (bytes + 255) & ~(size_t)255; return p; };
-  s.T = (int32_t *)take(s.arena_words * 4);
-  s.act = (unsigned long long *)take((size_t)s.nact * c->n * 8);
-  s.lst = (int32_t *)take((size_t)s.nlst * c->n * 4);
-  s.blst = (int32_t *)take((size_t)s.nblst * c->bcap * 8);
-  s.tab = (int32_t *)take((size_t)(MAXK * MAXK * MAXW + MAXK * MAXW + MAXK * MAXK) * 4);
-  s.stk = (int32_t *)take((size_t)SCAP * 12);
-  s.out = (int32_t *)take((size_t)ECAP * 12);
-  s.stx = (unsigned long long *)take((size_t)NSTRIPE * NSTX * 8);
-  if (off > s.bytes) {  // alignment slack
-    (void)hipFree(base);
-    s.bytes = off + 4096;
-    e = hipMalloc((void **)&base, s.bytes);
-    if (e != hipSuccess) { (void)hipGetLastError(); set_err("hipMalloc(slot) failed"); s = Slot(); return GST_OVER_BUDGET; }
-    off = 0;
-    s.T = (int32_t *)take(s.arena_words * 4);
-    s.act = (unsigned long long *)take((size_t)s.nact * c->n * 8);
-    s.lst = (int32_t *)take((size_t)s.nlst * c->n * 4);
-    s.blst = (int32_t *)take((size_t)s.nblst * c->bcap * 8);
-    s.tab = (int32_t *)take((size_t)(MAXK * MAXK * MAXW + MAXK * MAXW + MAXK * MAXK) * 4);
-    s.stk = (int32_t *)take((size_t)SCAP * 12);
-    s.out = (int32_t *)take((size_t)ECAP * 12);
-    s.stx = (unsigned long long *)take((size_t)NSTRIPE * NSTX * 8);
-  }
+  slot_carve(c, s, base, K, D, ECAP);
   return GST_OK;
 }
 
@@ -215,21 +208,21 @@ int plain_slot(gst_ctx *c, int K, int D) {
   return GST_OK;
 }
 
-// Validated, de-duplicated query.
+// Validated, de-duplicated query (fixed capacity: no heap use).
 struct Query {
-  std::vector<int32_t> gids;
+  int32_t gids[MAXK] = {0};
   int K = 0, D = -1, flags = 0;
 };
 
 int check_query(const gst_ctx *c, const int32_t *gids, int32_t k, int32_t D, int32_t flags, Query &q) {
   if (!gids || k < 1) { set_err("query needs k >= 1 group ids"); return GST_BAD_INPUT; }
-  q.gids.clear();
+  q.K = 0;
   for (int i = 0; i < k; i++) {
     if (gids[i] < 0 || gids[i] >= c->n_groups) { set_err("group id out of range"); return GST_BAD_INPUT; }
-    if (std::find(q.gids.begin(), q.gids.end(), gids[i]) == q.gids.end()) q.gids.push_back(gids[i]);
+    if (std::find(q.gids, q.gids + q.K, gids[i]) != q.gids + q.K) continue;
+    if (q.K == MAXK) { set_err("k > GST_MAX_K"); return GST_BAD_INPUT; }
+    q.gids[q.K++] = gids[i];
   }
-  q.K = (int)q.gids.size();
-  if (q.K > MAXK) { set_err("k > GST_MAX_K"); return GST_BAD_INPUT; }
   if (D > GST_MAX_D) { set_err("D > GST_MAX_D"); return GST_BAD_INPUT; }
   q.D = D < 0 ? -1 : D;
   q.flags = flags | (c->opts.flags & ~GST_O_KEEP_ORDER);
@@ -240,7 +233,7 @@ void fill_jobconst(const gst_ctx *c, const Slot &s, const Query &q, JobConst &jc
   std::memset(&jc, 0, sizeof(jc));
   jc.T = s.T; jc.stx = s.stx; jc.act = s.act; jc.lst = s.lst; jc.blst = s.blst; jc.tab = s.tab;
   jc.stk = s.stk; jc.out = s.out;
-  jc.bcap = c->bcap; jc.scap = SCAP; jc.ecap = ECAP;
+  jc.bcap = c->bcap; jc.scap = SCAP; jc.ecap = s.ecap;
   jc.K = q.K; jc.D = q.D; jc.H = q.D >= 0 ? q.D + 1 : 1; jc.W = 1 << q.K; jc.R = jc.W - 1;
   jc.hop = q.D >= 0; jc.flags = q.flags; jc.nl = s.nlst;
   jc.delta = q.D >= 0 ? 0 : c->delta;
@@ -319,24 +312,25 @@ void fold_stats(const unsigned long long *stripes, JobState &js) {
 // Some connected component holds a member of every query group (the non-hop
 // feasibility condition; a diameter bound can only make a feasible query infeasible).
 bool comp_feasible(const gst_ctx *c, const Query &q) {
-  std::vector<int32_t> common;
-  for (size_t i = 0; i < q.gids.size(); i++) {
-    std::vector<int32_t> cs;
-    for (int64_t t = c->h_goffs[q.gids[i]]; t < c->h_goffs[q.gids[i] + 1]; t++) cs.push_back(c->comp[c->h_gverts[t]]);
-    std::sort(cs.begin(), cs.end());
-    cs.erase(std::unique(cs.begin(), cs.end()), cs.end());
-    if (i == 0) common.swap(cs);
-    else {
-      std::vector<int32_t> x;
-      std::set_intersection(common.begin(), common.end(), cs.begin(), cs.end(), std::back_inserter(x));
-      common.swap(x);
-    }
-    if (common.empty()) return false;
+  auto has_member_in = [&](int grp, int32_t comp) {
+    for (int64_t t = c->h_goffs[grp]; t < c->h_goffs[grp + 1]; t++)
+      if (c->comp[c->h_gverts[t]] == comp) return true;
+    return false;
+  };
+  for (int64_t t = c->h_goffs[q.gids[0]]; t < c->h_goffs[q.gids[0] + 1]; t++) {
+    const int32_t cc = c->comp[c->h_gverts[t]];
+    bool all = true;
+    for (int i = 1; i < q.K && all; i++) all = has_member_in(q.gids[i], cc);
+    if (all) return true;
   }
-  return true;
+  return false;
 }
 
-void finish_result(const gst_ctx *c, const Query &q, const JobState &js, const std::vector<int32_t> &edges,
+// Fills out from a finished job. ebuf holds n_raw tree edges (u, v, w) in device ids; they
+// are rewritten in place as the distinct edges in the caller's ids, sorted, and out->edges
+// takes ebuf when own_edges (freed by gst_result_free), else stays null (the caller's
+// buffer of gst_solve_noalloc).
+void finish_result(const gst_ctx *c, const Query &q, const JobState &js, int32_t *ebuf, int n_raw, bool own_edges,
                    gst_result *out) {
   std::memset(out, 0, sizeof(*out));
   out->k = q.K;
@@ -360,6 +354,8 @@ void finish_result(const gst_ctx *c, const Query &q, const JobState &js, const s
   out->lb = -1;
   const int H = q.D >= 0 ? q.D + 1 : 1;
   auto ext = [&](int32_t v) { return c->old_of_new.empty() ? v : c->old_of_new[v]; };
+  const bool has_tree = js.ans != ~0ull && !js.err;
+  if (!has_tree && own_edges && ebuf) std::free(ebuf);
   if (js.err == 7 && js.ans != ~0ull) {
     // the optimum is known; only its tree exceeds the edge buffer
     const long long idx = (long long)(js.ans & 0xffffffffull);
@@ -391,30 +387,32 @@ void finish_result(const gst_ctx *c, const Query &q, const JobState &js, const s
   out->root = ext((int32_t)(idx / H));
   out->root_height = q.D >= 0 ? (int32_t)(idx % H) : -1;
   // distinct edges in the caller's ids, sorted
-  std::vector<std::array<int32_t, 3>> e;
-  for (size_t i = 0; i + 2 < edges.size(); i += 3) {
-    const int32_t a = ext(edges[i]), b = ext(edges[i + 1]);
-    e.push_back({std::min(a, b), std::max(a, b), edges[i + 2]});
+  struct E { int32_t a, b, w; };
+  E *e = reinterpret_cast<E *>(ebuf);
+  const int nr = ebuf ? n_raw : 0;
+  for (int i = 0; i < nr; i++) {
+    const int32_t a = ext(e[i].a), b = ext(e[i].b);
+    e[i].a = std::min(a, b);
+    e[i].b = std::max(a, b);
   }
-  std::sort(e.begin(), e.end());
-  e.erase(std::unique(e.begin(), e.end(), [](const std::array<int32_t, 3> &a, const std::array<int32_t, 3> &b) {
-            return a[0] == b[0] && a[1] == b[1];
-          }), e.end());
-  out->n_edges = (int32_t)e.size();
+  std::sort(e, e + nr, [](const E &x, const E &y) { return x.a != y.a ? x.a < y.a : x.b != y.b ? x.b < y.b : x.w < y.w; });
+  int nu = 0;
   long long sum = 0;
-  if (!e.empty()) {
-    out->edges = (int32_t *)std::malloc(sizeof(int32_t) * 3 * e.size());
-    for (size_t i = 0; i < e.size(); i++) {
-      out->edges[3 * i] = e[i][0]; out->edges[3 * i + 1] = e[i][1]; out->edges[3 * i + 2] = e[i][2];
-      sum += e[i][2];
-    }
+  for (int i = 0; i < nr; i++) {
+    if (nu && e[nu - 1].a == e[i].a && e[nu - 1].b == e[i].b) continue;
+    e[nu++] = e[i];
+    sum += e[i].w;
   }
+  out->n_edges = nu;
+  out->edges = (own_edges && nu) ? ebuf : nullptr;
+  if (own_edges && ebuf && !nu) std::free(ebuf);
   out->tree_cost = (int32_t)sum;
 }
 
 }  // namespace
 
 #include "gst_batch.inc"
+#include "gst_cohort.inc"
 
 namespace {
 
@@ -486,7 +484,7 @@ DeviceGraph renumber(const gst_csr *csr, const gst_groups *groups, bool reorder)
 
 extern "C" {
 
-const char *gst_last_error(void) { return g_err.c_str(); }
+const char *gst_last_error(void) { return g_err; }
 
 const char *gst_strerror(int code) {
   switch (code) {
@@ -615,6 +613,9 @@ int gst_open(const gst_csr *csr, const gst_groups *groups, const gst_opts *opts,
   HIPCHK(hipMalloc((void **)&c->d_jc, sizeof(JobConst) * c->max_batch));
   HIPCHK(hipMalloc((void **)&c->d_js, sizeof(JobState) * c->max_batch));
   HIPCHK(hipHostMalloc((void **)&c->h_js, sizeof(JobState) * c->max_batch, hipHostMallocDefault));
+  HIPCHK(hipHostMalloc((void **)&c->h_jc, sizeof(JobConst), hipHostMallocDefault));
+  HIPCHK(hipHostMalloc((void **)&c->h_stx, sizeof(unsigned long long) * NSTRIPE * NSTX, hipHostMallocDefault));
+  HIPCHK(hipHostMalloc((void **)&c->h_count, sizeof(unsigned long long), hipHostMallocDefault));
   HIPCHK(hipMalloc((void **)&c->d_count, sizeof(unsigned long long)));
   HIPCHK(hipMalloc((void **)&c->d_plan, sizeof(Plan)));
   HIPCHK(hipEventCreate(&c->ev0));
@@ -676,7 +677,7 @@ int gst_set_tuning(gst_ctx *c, int32_t degree_split, int32_t chunk_edges, int32_
   return GST_OK;
 }
 
-static int solve_one(gst_ctx *c, const int32_t *gids, int32_t k, int32_t D, int32_t flags, gst_result *out);
+static int solve_on_slot(gst_ctx *c, Slot &s, const Query &q, int32_t *ebuf, int32_t ecap_caller, gst_result *out);
 
 int gst_solve(gst_ctx *c, const int32_t *gids, int32_t k, int32_t D, int32_t flags, gst_result *out) {
   if (!c || !out) { set_err("null argument"); return GST_BAD_INPUT; }
@@ -686,7 +687,14 @@ int gst_solve(gst_ctx *c, const int32_t *gids, int32_t k, int32_t D, int32_t fla
   out->status = GST_DEVICE_ERROR;
   out->cost = -1;
   out->ub = out->lb = -1;
-  const int rc = solve_one(c, gids, k, D, flags, out);
+  int rc;
+  {
+    std::lock_guard<std::mutex> lk(c->solve_mu);
+    Query q;
+    rc = check_query(c, gids, k, D, flags, q);
+    if (rc == GST_OK) rc = hipSetDevice(c->dev) == hipSuccess ? plain_slot(c, q.K, q.D) : GST_DEVICE_ERROR;
+    if (rc == GST_OK) rc = solve_on_slot(c, c->plain.front().second, q, nullptr, 0, out);
+  }
   if (rc < 0 && rc != GST_INTERNAL && rc != GST_COST_RANGE) {
     gst_result_free(out);
     out->n_edges = 0;
@@ -696,19 +704,14 @@ int gst_solve(gst_ctx *c, const int32_t *gids, int32_t k, int32_t D, int32_t fla
   return rc;
 }
 
-static int solve_one(gst_ctx *c, const int32_t *gids, int32_t k, int32_t D, int32_t flags, gst_result *out) {
-  std::lock_guard<std::mutex> lk(c->solve_mu);
+// One query on slot s, host-driven rounds. With ebuf == nullptr the tree goes to a fresh
+// heap buffer owned by the result; otherwise into ebuf (ecap_caller triples), with no
+// host or device allocation on the way.
+static int solve_on_slot(gst_ctx *c, Slot &s, const Query &q, int32_t *ebuf, int32_t ecap_caller, gst_result *out) {
   const auto t0 = clk::now();
-  Query q;
-  int rc = check_query(c, gids, k, D, flags, q);
-  if (rc) { out->status = rc; out->cost = -1; return rc; }
-  HIPCHK(hipSetDevice(c->dev));
-  rc = plain_slot(c, q.K, q.D);
-  if (rc) { out->status = rc; out->cost = -1; return rc; }
-  Slot &s = c->plain.front().second;
-  JobConst jc;
+  JobConst &jc = *c->h_jc;
   fill_jobconst(c, s, q, jc);
-  JobState js;
+  JobState &js = c->h_js[0];
   init_jobstate(js);
   HIPCHK(hipMemcpyAsync(c->d_jc, &jc, sizeof(jc), hipMemcpyHostToDevice, c->st));
   HIPCHK(hipMemcpyAsync(c->d_js, &js, sizeof(js), hipMemcpyHostToDevice, c->st));
@@ -791,25 +794,30 @@ static int solve_one(gst_ctx *c, const int32_t *gids, int32_t k, int32_t D, int3
     // DP-loop time: merge and grow rounds after the level-1 SSSP (hop: every round)
     dp_round = (ph == PH_MERGE || ph == PH_GROW || ph == PH_HOPCLEAR) && (q.D >= 0 || h.level >= 2);
   }
-  std::vector<int32_t> edges;
-  if (h.n_out > 0 && h.ans != ~0ull && !h.err) {
-    edges.resize((size_t)3 * std::min(h.n_out, ECAP));
-    HIPCHK(hipMemcpy(edges.data(), s.out, sizeof(int32_t) * edges.size(), hipMemcpyDeviceToHost));
+  const int n_out = std::min(std::max(h.n_out, 0), s.ecap);
+  int32_t *eb = nullptr;
+  if (n_out > 0 && h.ans != ~0ull && !h.err) {
+    eb = ebuf ? ebuf : (int32_t *)std::malloc(sizeof(int32_t) * 3 * n_out);
+    if (ebuf && n_out > ecap_caller) { set_err("tree exceeds edge_cap"); return GST_INTERNAL; }
+    const hipError_t e = hipMemcpy(eb, s.out, sizeof(int32_t) * 3 * n_out, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) {
+      if (!ebuf) std::free(eb);
+      std::snprintf(g_err, sizeof g_err, "tree copy: %s", hipGetErrorString(e));
+      return GST_DEVICE_ERROR;
+    }
   }
   JobState fin = h;
-  std::vector<unsigned long long> stripes((size_t)NSTRIPE * NSTX);
-  HIPCHK(hipMemcpy(stripes.data(), s.stx, stripes.size() * 8, hipMemcpyDeviceToHost));
-  fold_stats(stripes.data(), fin);
-  finish_result(c, q, fin, edges, out);
+  HIPCHK(hipMemcpy(c->h_stx, s.stx, (size_t)NSTRIPE * NSTX * 8, hipMemcpyDeviceToHost));
+  fold_stats(c->h_stx, fin);
+  finish_result(c, q, fin, eb, n_out, ebuf == nullptr, out);
   if (q.flags & GST_F_STATS) {
     k_fill_u64<<<1, 64, 0, c->st>>>(c->d_count, 1, 0ull);
     // the states of levels 1..k (the arena's last n * H words are unused and not filled)
     const long long used = std::min<long long>((long long)s.arena_words, jc.off[q.K + 1]);
     k_count_finite<<<grid_for(used, BLK, 4096), BLK, 0, c->st>>>(s.T, used, c->d_count);
-    unsigned long long cnt = 0;
-    HIPCHK(hipMemcpyAsync(&cnt, c->d_count, sizeof(cnt), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipMemcpyAsync(c->h_count, c->d_count, sizeof(unsigned long long), hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
-    out->states_finite = (int64_t)cnt;
+    out->states_finite = (int64_t)*c->h_count;
   }
   out->t_dp_s = t_dp;
   out->t_grow_s = t_grow;
@@ -841,6 +849,7 @@ void gst_close(gst_ctx *c) {
   (void)hipFree(c->d_beg); (void)hipFree(c->d_adj); (void)hipFree(c->d_w);
   (void)hipFree(c->d_goffs); (void)hipFree(c->d_gverts); (void)hipFree(c->d_nchunk);
   (void)hipFree(c->d_jc); (void)hipFree(c->d_js); (void)hipHostFree(c->h_js);
+  (void)hipHostFree(c->h_jc); (void)hipHostFree(c->h_stx); (void)hipHostFree(c->h_count);
   (void)hipFree(c->d_count);
   (void)hipFree(c->d_plan);
   if (c->ev0) (void)hipEventDestroy(c->ev0);

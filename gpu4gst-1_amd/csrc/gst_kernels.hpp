@@ -92,6 +92,15 @@ struct JobConst {
   int32_t seed_total;         // members of the query groups (seed work items)
   int32_t tv;                 // vertices per merge tile
   int32_t delta;              // near-far step of the grow frontier (0 = off)
+  int32_t passive;            // cohort member: the grow phases run in the cohort kernels
+  // cohort member only: the merge of a level L < K writes its states into the cohort's
+  // level block cT (row of vertex v at v * S, S = cohort_stride(cB, C(k,L)), this member's
+  // labels at cq * C(k,L)) and activates them in list 1 of the cohort's label masks
+  // (cmask: [2][n][S / 64]) and vertex flags (cflag: [2][n])
+  int32_t *cT;
+  unsigned long long *cmask;
+  int32_t *cflag;
+  int32_t cB, cq;             // members in the cohort, this member's index
   long long off[9];           // arena block offset of each level (subset size)
   int32_t gid[8];
   uint8_t nsub[8];
@@ -173,6 +182,9 @@ __device__ __forceinline__ gi32 *as_global(int32_t *p) { return (gi32 *)p; }
 __device__ __forceinline__ const gi32 *as_global(const int32_t *p) { return (const gi32 *)p; }
 __device__ __forceinline__ gu64 *as_global(unsigned long long *p) { return (gu64 *)p; }
 __device__ __forceinline__ const gu64 *as_global(const unsigned long long *p) { return (const gu64 *)p; }
+
+// Words of a cohort row: B members' C(k,L) labels, padded to whole 64-label chunks.
+__host__ __device__ __forceinline__ int cohort_stride(int B, int C) { return ((B * C + 63) / 64) * 64; }
 
 // Arena index of state (v, p, d).
 __device__ __forceinline__ size_t sidx(const JobConst &c, int32_t v, int p, int d) {
@@ -281,6 +293,7 @@ __device__ void step_job(const JobConst &c, JobState &s) {
   int ph = s.phase;
   if (ph == PH_DONE) return;
   s.rounds++;
+  if (c.passive) return;  // a cohort member: its driver sets the phase of every round
   switch (ph) {
     case PH_INIT:
       s.level = 1; s.depth = 0; s.parity = 0;
@@ -460,12 +473,11 @@ __global__ __launch_bounds__(BLK) void k_seed(const JobConst *__restrict__ jcs, 
 // Evaluate the active states of v (bits of a) at the current level: update UB, and
 // return the subset of bits that may grow. Non-hop jobs (grow_select_hop for hop jobs).
 // Non-hop: v's active states of level L are bits of a (bit j = offset j of v's level row).
-__device__ __forceinline__ unsigned long long grow_select(const JobConst &c, JobState *s,
-                                                          int32_t v, unsigned long long a,
-                                                          int L, int ub, int &n_states, int theta,
-                                                          unsigned long long &defer, int &dmin) {
-  const int32_t *row = lrow(c, L, v, 0);
-  const int32_t *sing = lrow(c, 1, v, 0);
+// row: v's states of level L (bit j of a = row[j]); sing: v's singleton states.
+__device__ __forceinline__ unsigned long long grow_select_rows(const JobConst &c, JobState *s, int32_t v,
+                                                               const int32_t *row, const int32_t *sing,
+                                                               unsigned long long a, int L, int ub, int &n_states,
+                                                               int theta, unsigned long long &defer, int &dmin) {
   const int32_t *hk = c.tab;
   const int32_t *w2 = c.tab + c.K * c.K * c.W;
   const bool prune = !(c.flags & JF_NO_PRUNE);
@@ -503,6 +515,13 @@ __device__ __forceinline__ unsigned long long grow_select(const JobConst &c, Job
     gm |= 1ull << j;
   }
   return gm;
+}
+
+__device__ __forceinline__ unsigned long long grow_select(const JobConst &c, JobState *s,
+                                                          int32_t v, unsigned long long a,
+                                                          int L, int ub, int &n_states, int theta,
+                                                          unsigned long long &defer, int &dmin) {
+  return grow_select_rows(c, s, v, lrow(c, L, v, 0), lrow(c, 1, v, 0), a, L, ub, n_states, theta, defer, dmin);
 }
 
 // Diameter-bounded variant: evaluate the active states of (v, level L, height d).
@@ -1479,7 +1498,10 @@ __global__ __launch_bounds__(BLK) void k_merge(const JobConst *__restrict__ jcs,
       }
       // every state of the level is written here, INF included: levels >= 2 are not
       // filled at query start
-      as_global(c.T)[(size_t)c.off[L] + (size_t)(v0 + vi) * nL + j] = m < INF ? m : INF;
+      if (c.passive && L < K)
+        as_global(c.cT)[(size_t)(v0 + vi) * cohort_stride(c.cB, nL) + c.cq * nL + j] = m < INF ? m : INF;
+      else
+        as_global(c.T)[(size_t)c.off[L] + (size_t)(v0 + vi) * nL + j] = m < INF ? m : INF;
       if (m >= INF) continue;
       if (m >= SAT_AT) flag_saturation(s);
       merged++;
@@ -1503,8 +1525,20 @@ __global__ __launch_bounds__(BLK) void k_merge(const JobConst *__restrict__ jcs,
     __syncthreads();
     if (threadIdx.x == 0 && ubs < ub) atomicMin(&s->ub, ubs);
     if (threadIdx.x == 0) minc_acc = minc_s < minc_acc ? minc_s : minc_acc;
-    for (int vi = threadIdx.x; vi < tv; vi += BLK)  // k_collect builds the worklist
-      if (amask[vi]) c.act[(size_t)lo * g.n + v0 + vi] = amask[vi];
+    if (c.passive) {  // a cohort member: activate in list 1 of the cohort's label masks
+      const int S = cohort_stride(c.cB, nL), l0 = c.cq * nL, w0 = l0 >> 6, sh = l0 & 63;
+      for (int vi = threadIdx.x; vi < tv; vi += BLK) {
+        const unsigned long long a = amask[vi];
+        if (!a) continue;
+        gu64 *mw = as_global(c.cmask) + ((size_t)g.n + v0 + vi) * (S >> 6) + w0;
+        __hip_atomic_fetch_or(mw, a << sh, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (sh + nL > 64) __hip_atomic_fetch_or(mw + 1, a >> (64 - sh), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        as_global(c.cflag)[(size_t)g.n + v0 + vi] = 1;
+      }
+    } else {
+      for (int vi = threadIdx.x; vi < tv; vi += BLK)  // k_collect builds the worklist
+        if (amask[vi]) c.act[(size_t)lo * g.n + v0 + vi] = amask[vi];
+    }
     __syncthreads();
   }
   merged_acc += merged;

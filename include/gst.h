@@ -106,6 +106,9 @@ typedef struct {
   int32_t near_far_delta;  /* near-far step of the non-hop grow frontier: states costlier
                               than the threshold wait until the near frontier drains
                               (<= 0 = off: plain label-correcting rounds) */
+  int32_t cohort_max;      /* batched/served forms: queued non-hop queries of one k are
+                              grown together, up to this many at a time (0 = up to 64, as
+                              many as fit; 1 = each query on its own) */
 } gst_opts;
 
 typedef struct {

@@ -57,15 +57,18 @@ def samples(wl):
         return {"plain": [(x, -1) for x in q[5][:8]],
                 "mixed": [(x, -1) for x in list(q[3][:6]) + list(q[5][8:13]) + list(q[7][:5])]}
     if wl.name == "cl4m":
-        # hop queries at k = 7 need tens of GB in the oracle: they are checked for
-        # "alone == in the mix" only, and carry no expected cost here
+        # hop queries at k = 7 need more than ORACLE_BYTES in the oracle: they are checked
+        # for "alone == in the mix" only, and carry no expected cost here
         mix = served_mix(wl, 24, datasets.SEED_BASE + 4 + 555)
         return {"served": [(x, D) for x, D in mix]}
     raise KeyError(wl.name)
 
 
-def oracle_ok(k, D):
-    return D < 0 or k <= 5
+ORACLE_BYTES = 40e9  # host memory one oracle query may take (its flat state arrays)
+
+
+def oracle_bytes(n, k, D):
+    return n * (1 << k) * (D + 1 if D >= 0 else 1) * (24 if D >= 0 else 16)
 
 
 def run(name, threads):
@@ -84,10 +87,10 @@ def run(name, threads):
             groups.setdefault((len(x), D), []).append(i)
         res = {}
         for (k, D), idx in sorted(groups.items()):
-            if not oracle_ok(k, D):
+            per_query = oracle_bytes(wl.graph.n, k, D)
+            if per_query > ORACLE_BYTES:
                 continue
-            per_query = wl.graph.n * (1 << k) * (D + 1 if D >= 0 else 1) * (24 if D >= 0 else 16)
-            nt = max(1, min(threads, len(idx), int(40e9 // per_query)))
+            nt = max(1, min(threads, len(idx), int(ORACLE_BYTES // per_query)))
             t = time.time()
             outs = pyoracle.batch(inst, np.stack([qs[i][0] for i in idx]), algo=0 if D < 0 else 1, D=max(D, 0),
                                   n_threads=nt)

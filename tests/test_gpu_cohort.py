@@ -1,0 +1,78 @@
+"""Cohorts: queued non-hop queries of one k grown together over a member-interleaved level
+block (the batched and served forms). Every member's result equals the same query solved
+alone and the CPU oracle; trees are valid; the finite-state count pins that every state
+of every member was written."""
+import numpy as np
+import pytest
+
+import pyoracle
+from gpu4gst_1_amd import engine
+
+pytestmark = pytest.mark.gpu
+
+
+def _queries(rmat, ks, per_k, seed, D=-1):
+    rng = np.random.default_rng(seed)
+    return [engine.Query(rng.choice(rmat.groups.n_groups, k, replace=False).astype(np.int32), D)
+            for k in ks for _ in range(per_k)]
+
+
+def _check(rmat, inst, q, r, alone):
+    o = pyoracle.prunedpp(inst, q.gids) if q.D < 0 else pyoracle.dprunedpp(inst, q.gids, q.D)
+    if o["status"] != 0:
+        assert r["status"] == alone["status"] == engine.GST_NO_SOLUTION, (q.gids, r["status"])
+        return
+    assert r["status"] == engine.GST_OK, (q.gids, q.D, r["status"])
+    assert r["cost"] == alone["cost"] == o["tree_cost"], (q.gids, q.D, r["cost"], alone["cost"], o["tree_cost"])
+    assert r["tree_cost"] == r["cost"]
+    ok, why = engine.validate_tree(rmat.graph, rmat.groups, q.gids, r["edges"], r["cost"], q.D)
+    assert ok, (q.gids, why)
+
+
+def test_cohort_k5(gpu_engine, rmat, rmat_inst):
+    qs = _queries(rmat, [5], 24, 2024)
+    print("cohort of", len(qs), "k=5 queries", flush=True)
+    res = gpu_engine.solve_batch(qs)
+    for q, r in zip(qs, res):
+        _check(rmat, rmat_inst, q, r, gpu_engine.solve(q.gids))
+
+
+def test_cohorts_mixed_k_and_hop(gpu_engine, rmat, rmat_inst):
+    """One batch: cohorts of k = 2..7 (k = 7 at up to 29 members per cohort, so 40 of
+    them form two cohorts) beside diameter-bounded queries, which run on their own."""
+    qs = _queries(rmat, [2, 3, 4, 6], 5, 7) + _queries(rmat, [7], 40, 8) + _queries(rmat, [3, 5], 4, 9, D=3)
+    rng = np.random.default_rng(10)
+    order = rng.permutation(len(qs))
+    qs = [qs[i] for i in order]
+    print("mixed batch of", len(qs), flush=True)
+    res = gpu_engine.solve_batch(qs)
+    for q, r in zip(qs, res):
+        _check(rmat, rmat_inst, q, r, gpu_engine.solve(q.gids, D=q.D))
+
+
+def test_cohort_equals_single_jobs(rmat):
+    """cohort_max = 1 turns cohorts off: the same batch through per-query jobs gives the
+    same costs; without pruning, the same trees (the arenas then hold the exact DP values
+    and the rebuild's choices are canonical)."""
+    qs = _queries(rmat, [3, 4, 5], 6, 31)
+    with engine.Engine(rmat.graph, rmat.groups, device=0) as a, \
+            engine.Engine(rmat.graph, rmat.groups, device=0, cohort_max=1) as b:
+        for flags in (0, engine.F_NO_PRUNE):
+            qf = [engine.Query(q.gids, q.D, flags) for q in qs]
+            print("cohort vs single, flags", flags, flush=True)
+            ra, rb = a.solve_batch(qf), b.solve_batch(qf)
+            for q, x, y in zip(qf, ra, rb):
+                assert x["status"] == y["status"] == engine.GST_OK and x["cost"] == y["cost"], (q.gids, x, y)
+                if flags:
+                    assert np.array_equal(x["edges"], y["edges"]), q.gids
+
+
+@pytest.mark.parametrize("k", [2, 4, 6])
+def test_cohort_stats_count_unpruned(gpu_engine, rmat, k):
+    """Unpruned, on the connected R-MAT graph, every state of every member is finite:
+    n (2^k - 1) per member (levels 1..k-1 come back from the cohort block)."""
+    qs = [engine.Query(q.gids, -1, engine.F_STATS | engine.F_NO_PRUNE) for q in _queries(rmat, [k], 5, 50 + k)]
+    res = gpu_engine.solve_batch(qs)
+    for q, r in zip(qs, res):
+        assert r["states_finite"] == rmat.graph.n * (2 ** k - 1), (q.gids, r["states_finite"])
+        assert r["cost"] == gpu_engine.solve(q.gids)["cost"]
