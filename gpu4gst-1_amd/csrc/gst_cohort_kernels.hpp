@@ -149,8 +149,9 @@ __global__ __launch_bounds__(BLK) void k_cselect(const CohortConst *__restrict__
       if (sh + C > 64) atomicAnd(&row_m[w0 + 1], ~(drop >> (64 - sh)));
     }
   }
-  for (int off = 32; off > 0; off >>= 1) n_states += __shfl_xor(n_states, off, 64);
-  if (lane == 0 && n_states) atomicAdd(cc.stx + (blockIdx.x & (NSTRIPE - 1)) * NSTX + ST_STATES, n_states);
+  // lane q counted member q's evaluated states over this wave's items
+  if (lane < cc.B && n_states)
+    atomicAdd(jcs[cc.jobs[lane]].stx + (blockIdx.x & (NSTRIPE - 1)) * NSTX + ST_STATES, n_states);
 }
 
 // Grow: a wavefront per item (vertex, chunk of arcs); lanes over the row's labels, 64 per
