@@ -113,6 +113,7 @@ struct gst_ctx {
   JobConst *h_jc = nullptr;  // pinned staging of the single-query path's job constants
   unsigned long long *h_stx = nullptr;    // pinned [NSTRIPE * NSTX] counter stripes
   unsigned long long *h_count = nullptr;  // pinned finite-state count
+  void *scratch = nullptr;                // this device's static scratch (gst_solve_noalloc)
   Plan *d_plan = nullptr;    // work layout of the single-query path
   unsigned long long *d_count = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;  // bracket the work kernels of one round
@@ -415,6 +416,10 @@ void finish_result(const gst_ctx *c, const Query &q, const JobState &js, int32_t
 #include "gst_cohort.inc"
 
 namespace {
+int noalloc_setup(gst_ctx *c);  // gst_noalloc.inc
+}
+
+namespace {
 
 // The graph as the device holds it. With renumbering, device vertex i is the caller's
 // vertex old_of_new[i], taken in order of descending degree (ties by id), and every
@@ -630,6 +635,8 @@ int gst_open(const gst_csr *csr, const gst_groups *groups, const gst_opts *opts,
   int rc = gst_set_tuning(c.get(), c->split, c->chunk, c->delta);
   if (rc) return rc;
   rc = batch_open(c.get());
+  if (rc) return rc;
+  rc = noalloc_setup(c.get());
   if (rc) return rc;
   *out = c.release();
   return GST_OK;
@@ -859,3 +866,5 @@ void gst_close(gst_ctx *c) {
 }
 
 }  // extern "C"
+
+#include "gst_noalloc.inc"

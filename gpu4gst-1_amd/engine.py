@@ -69,7 +69,7 @@ class GstInfo(ctypes.Structure):
 
 EXPORTS = ["gst_open", "gst_solve", "gst_solve_batch", "gst_submit", "gst_poll", "gst_wait", "gst_info_get",
            "gst_set_tuning", "gst_result_free", "gst_close", "gst_strerror", "gst_last_error", "gst_arena_bytes",
-           "gst_job_bytes"]
+           "gst_job_bytes", "gst_solve_noalloc"]
 
 
 def lib_path() -> str:
@@ -101,6 +101,7 @@ def lib():
         L.gst_last_error.restype = ctypes.c_char_p
         L.gst_arena_bytes.restype = ctypes.c_int64
         L.gst_arena_bytes.argtypes = [i32, i32, i32]
+        L.gst_solve_noalloc.argtypes = [P, P, i32, i32, i32, P, i32, ctypes.POINTER(GstResult)]
         L.gst_job_bytes.restype = ctypes.c_int64
         L.gst_job_bytes.argtypes = [P, i32, i32]
         _LIB = L
@@ -186,6 +187,23 @@ class Engine:
             return d
         finally:
             L.gst_result_free(ctypes.byref(r))
+
+    def solve_noalloc(self, gids, D=-1, flags=0, edge_cap=None) -> dict:
+        """gst_solve_noalloc: no device or host allocation in the call; the tree goes to a
+        caller buffer of edge_cap triples (default n - 1, which always suffices)."""
+        L = lib()
+        g = np.ascontiguousarray(gids, np.int32)
+        cap = max(self.n - 1, 1) if edge_cap is None else int(edge_cap)
+        buf = np.zeros(3 * max(cap, 1), np.int32)
+        r = GstResult()
+        rc = L.gst_solve_noalloc(self._h, g.ctypes.data, len(g), int(D), int(flags), buf.ctypes.data, cap,
+                                 ctypes.byref(r))
+        if rc < 0 and rc not in (GST_INTERNAL, GST_COST_RANGE):
+            raise GstError(rc, L.gst_last_error().decode())
+        d = _result_dict(r)
+        n = max(r.n_edges, 0) if r.status in (GST_OK,) else 0
+        d["edges"] = buf[:3 * n].reshape(n, 3).copy()
+        return d
 
     def DP_GPU(self, gids):
         """Non-hop solve with the reference driver's cost convention (0 = no solution)."""

@@ -179,6 +179,17 @@ void gst_close(gst_ctx *ctx);
 const char *gst_strerror(int code);
 const char *gst_last_error(void);
 int64_t gst_arena_bytes(int32_t n, int32_t k, int32_t D);
+/* No counterpart in the reference: for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid allocation). gst_solve without any
+ * device or host allocation: the query's DP memory comes from a static device array of
+ * GST_NOALLOC_MAX_BYTES per device (queries that need more return GST_OVER_BUDGET), and
+ * the tree's distinct edges go to the caller's edges[3 * edge_cap] (out->edges stays
+ * NULL; n - 1 edges always suffice; a larger tree returns GST_TREE_OMITTED with the cost).
+ * Calls through one context are serialised; calls through two contexts on one device at
+ * the same time race on the shared array. */
+#define GST_NOALLOC_MAX_BYTES ((int64_t)64 << 20)
+int gst_solve_noalloc(gst_ctx *ctx, const int32_t *gids, int32_t k, int32_t D, int32_t flags, int32_t *edges,
+                      int32_t edge_cap, gst_result *out);
 /* Device memory one admitted query of this shape holds in the batched and served forms
  * (arena, activation masks, worklists, tables, tree buffers): what admission charges
  * against dp_budget_bytes. < 0 on bad arguments. */
