@@ -164,6 +164,169 @@ def gather_ceiling():
     print(json.dumps({"gather_ceiling": rows}))
 
 
+# ---------------------------------------------------------------- HBM traffic
+TRAFFIC_KERNELS = ("k_select", "k_grow_thread", "k_grow_wave")  # one grow round (the roofline launch)
+COPY_MIB = 256
+
+
+def traffic_probe(shm, split, chunk, n_q):
+    """Child of a counter pass (run under rocprofv3 --pmc): one device-to-device copy of
+    COPY_MIB MiB (the calibration dispatch), then n_q plain solves of the workload saved in
+    shm. Prints the grow rounds of the solves as JSON."""
+    import ctypes
+
+    hip = ctypes.CDLL(os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib", "libamdhip64.so"))
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    nb = COPY_MIB << 20
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nb)) == 0 and hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nb)) == 0
+    assert hip.hipMemcpy(b, a, ctypes.c_size_t(nb), 3) == 0 and hip.hipDeviceSynchronize() == 0  # 3: device to device
+    hip.hipFree(a)
+    hip.hipFree(b)
+    ld = lambda name: np.load(os.path.join(shm, name + ".npy"), mmap_mode="r")
+    graph = datasets.Graph(np.asarray(ld("beg")), np.asarray(ld("adj")), np.asarray(ld("w")))
+    groups = datasets.Groups(np.asarray(ld("offs")), np.asarray(ld("verts")))
+    qs = np.asarray(ld("queries"))
+    eng = engine.Engine(graph, groups)
+    eng.set_tuning(split, chunk, 0)
+    rounds = [eng.solve(q)["grow_rounds"] for q in qs[:n_q]]
+    eng.close()
+    print(json.dumps({"grow_rounds": rounds}))
+
+
+def _counter_sums(outdir):
+    """Per kernel: (dispatches, summed counter value) from rocprofv3's counter CSV."""
+    import csv
+    import glob
+
+    files = glob.glob(os.path.join(outdir, "**", "*counter_collection.csv"), recursive=True)
+    per_dispatch = {}
+    for fn in files:
+        with open(fn) as f:
+            for row in csv.DictReader(f):
+                key = (row.get("Dispatch_Id") or row.get("Correlation_Id"), row["Kernel_Name"])
+                per_dispatch[key] = per_dispatch.get(key, 0.0) + float(row["Counter_Value"])
+    out = {}
+    for (_, name), v in per_dispatch.items():
+        n, tot, mx = out.get(name, (0, 0.0, 0.0))
+        out[name] = (n + 1, tot + v, max(mx, v))
+    return out
+
+
+def measure_traffic(wl, queries, split, chunk, log):
+    """HBM bytes per grow round of the plain path from two counter passes (FETCH_SIZE, then
+    WRITE_SIZE, KiB per dispatch), each scaled by what the same counter reads for the copy
+    dispatch of known size in the same process. Runs before this process touches the GPU
+    (the passes are child processes). Returns a dict, or None with the reason logged."""
+    import shutil
+    import subprocess
+    import tempfile
+
+    if not shutil.which("rocprofv3"):
+        log("# traffic: rocprofv3 not found")
+        return None
+    shm = tempfile.mkdtemp(prefix="gst_bench_", dir="/dev/shm" if os.path.isdir("/dev/shm") else None)
+    try:
+        for name, arr in (("beg", wl.graph.beg_pos), ("adj", wl.graph.adj), ("w", wl.graph.w),
+                          ("offs", wl.groups.offs), ("verts", wl.groups.verts), ("queries", np.stack(queries))):
+            np.save(os.path.join(shm, name + ".npy"), np.ascontiguousarray(arr))
+        res = {}
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            out = os.path.join(shm, counter)
+            cmd = ["rocprofv3", "--pmc", counter, "--output-format", "csv", "-d", out, "-o", "pass", "--",
+                   sys.executable, os.path.abspath(__file__), "--traffic-probe", shm, str(split), str(chunk)]
+            env = dict(os.environ, TMPDIR="/tmp")
+            t = time.time()
+            r = subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                               text=True, timeout=240)
+            lines = [x for x in r.stdout.splitlines() if x.startswith("{")]
+            if not lines:
+                log(f"# traffic: {counter} pass failed (rc {r.returncode}): {r.stdout[-400:]}")
+                return None
+            rounds = sum(json.loads(lines[-1])["grow_rounds"])
+            sums = _counter_sums(out)
+            copy = [v for k, v in sums.items() if "copyBuffer" in k]
+            grow = {k: v for k, v in sums.items() if k.split("(")[0].split("::")[-1] in TRAFFIC_KERNELS}
+            n_sel = sum(v[0] for k, v in grow.items() if "k_select" in k)
+            if not copy or not grow or n_sel != rounds:
+                log(f"# traffic: {counter} pass unusable (copy {bool(copy)}, grow kernels {len(grow)}, "
+                    f"select dispatches {n_sel} vs rounds {rounds})")
+                return None
+            factor = (COPY_MIB << 10) / max(v[2] for v in copy)  # KiB the copy moved / KiB counted
+            kib = sum(v[1] for v in grow.values())
+            res[counter] = {"kib_per_round": kib / rounds, "copy_factor": round(factor, 4), "rounds": rounds}
+            log(f"# traffic: {counter} {kib / rounds / 1024:.1f} MiB per round counted, copy factor {factor:.3f} "
+                f"({time.time() - t:.0f}s)")
+        fetch = res["FETCH_SIZE"]["kib_per_round"] * res["FETCH_SIZE"]["copy_factor"] * 1024
+        write = res["WRITE_SIZE"]["kib_per_round"] * res["WRITE_SIZE"]["copy_factor"] * 1024
+        return {"bytes_per_round": int(fetch + write), "read_bytes_per_round": int(fetch),
+                "write_bytes_per_round": int(write), "passes": res,
+                "tuning": {"degree_split": split, "chunk_edges": chunk}}
+    except Exception as e:  # the bench line still prints, with traffic null
+        log(f"# traffic: {type(e).__name__}: {e}")
+        return None
+    finally:
+        shutil.rmtree(shm, ignore_errors=True)
+
+
+# ---------------------------------------------------------------- extra legs
+def hop_leg(log, k=5, D=4, n_q=16):
+    """Diameter-bounded plain path on the 1.1M-vertex Chung-Lu graph (the hop workload):
+    q/s and p50/p99 over n_q queries of k groups with bound D (the reference times its
+    persistent hop solve per query, code/D-TrimCDP-WB/src/GPUHop.cu:215-219)."""
+    wl = datasets.make_workload("cl1m")
+    with engine.Engine(wl.graph, wl.groups) as eng:
+        qs = wl.queries[k][:n_q]
+        eng.solve(qs[0], D=D)
+        res = [eng.solve(q, D=D) for q in qs]
+    lat = np.array([r["t_total_s"] for r in res]) * 1e3
+    out = {"workload": f"cl1m: Chung-Lu n={wl.graph.n}, undirected edges={wl.graph.n_edges}, "
+                       f"{wl.groups.n_groups} groups, k={k}, D={D}",
+           "qps": round(len(res) / (lat.sum() / 1e3), 3), "p50_ms": round(float(np.median(lat)), 3),
+           "p99_ms": round(float(np.percentile(lat, 99)), 3), "queries": len(res),
+           "solved": sum(r["status"] == engine.GST_OK for r in res)}
+    log(f"# hop leg: {out}")
+    return out
+
+
+def served_leg(log, rate=16.0, n_q=48, budget_gb=200.0, slots=64):
+    """Served form on the 4.8M-vertex Chung-Lu graph: Poisson arrivals at `rate` q/s with k
+    uniform in {3,5,7} and D uniform in {none,3,4,5}, admitted against a DP budget; q/s and
+    p50/p99 latency from arrival to result, beside the plain path on the same queries, and
+    every served result compared with the same query solved alone."""
+    wl = datasets.make_workload("cl4m")
+    rng = np.random.default_rng(datasets.SEED_BASE + 4 + 777)
+    qs = []
+    for _ in range(n_q):
+        k = int(rng.choice([3, 5, 7]))
+        D = int(rng.choice([-1, 3, 4, 5]))
+        qs.append(engine.Query(rng.choice(wl.groups.n_groups, k, replace=False).astype(np.int32), D))
+    with engine.Engine(wl.graph, wl.groups, max_batch=slots, dp_budget_bytes=int(budget_gb * 1e9)) as eng:
+        eng.solve(qs[0].gids, D=qs[0].D)
+        tp = time.perf_counter()
+        alone = [eng.solve(q.gids, D=q.D) for q in qs]
+        plain_dt = time.perf_counter() - tp
+        gaps = rng.exponential(1.0 / rate, size=n_q)
+        tickets = []
+        start = time.perf_counter()
+        nxt = start
+        for q, gap in zip(qs, gaps):
+            nxt += gap
+            while time.perf_counter() < nxt:
+                time.sleep(min(0.0005, max(0.0, nxt - time.perf_counter())))
+            tickets += eng.submit([q])
+        res = [eng.wait(t) for t in tickets]
+        dt = time.perf_counter() - start
+    lat = np.array([r["t_total_s"] for r in res]) * 1e3
+    same = sum(1 for r, a in zip(res, alone) if r["status"] == a["status"] and r["cost"] == a["cost"])
+    out = {"workload": f"cl4m: Chung-Lu n={wl.graph.n}, undirected edges={wl.graph.n_edges}, "
+                       f"{wl.groups.n_groups} groups, k in {{3,5,7}}, D in {{none,3,4,5}}",
+           "arrival_rate_qps": rate, "qps": round(n_q / dt, 3), "p50_ms": round(float(np.median(lat)), 2),
+           "p99_ms": round(float(np.percentile(lat, 99)), 2), "plain_qps": round(n_q / plain_dt, 3),
+           "equal_alone": f"{same}/{n_q}", "dp_budget_gb": budget_gb, "slots": slots}
+    log(f"# served leg: {out}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -189,7 +352,12 @@ def main():
                     help="time independent random 4-B reads (the ceiling of the grow round's access pattern)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device once (HBM counter calibration)")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 counter passes")
+    ap.add_argument("--no-legs", action="store_true", help="skip the hop and served legs")
+    ap.add_argument("--traffic-probe", nargs=3, metavar=("SHM", "SPLIT", "CHUNK"), help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.traffic_probe:
+        return traffic_probe(args.traffic_probe[0], int(args.traffic_probe[1]), int(args.traffic_probe[2]), 3)
     if args.calibrate_copy:
         return calibrate_copy(args.calibrate_copy)
     if args.gather_ceiling:
@@ -202,14 +370,22 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
                                    "(the path shards by query, see the design notes)"}))
         sys.exit(2)
-    import torch
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
 
-    assert torch.cuda.is_available(), "bench needs a GPU"
     t0 = time.time()
     wl = datasets.make_workload(args.workload)
     g = wl.graph
-    print(f"# workload {args.workload}: n={g.n} edges={g.n_edges} groups={wl.groups.n_groups} "
-          f"(generated in {time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
+    log(f"# workload {args.workload}: n={g.n} edges={g.n_edges} groups={wl.groups.n_groups} "
+        f"(generated in {time.time() - t0:.1f}s)")
+    # counter passes first: they are child processes, started before this one uses the GPU
+    traffic = None
+    if not args.no_traffic and not args.profile_only:
+        split, chunk = args.tuning if args.tuning else (32, 1024)
+        traffic = measure_traffic(wl, wl.queries[args.k][-3:], split, chunk, log)
+    import torch
+
+    assert torch.cuda.is_available(), "bench needs a GPU"
     need = args.batch * (args.steps + args.warmup)
     n_tune = 32  # queries after the batches, used only by the autotuner
     rng = np.random.default_rng(datasets.SEED_BASE + 77)
@@ -288,6 +464,28 @@ def main():
     plain_match = sum(1 for q, r in zip(plain_q, plain)
                       if tuple(q.tolist()) in by_q and by_q[tuple(q.tolist())]["cost"] == r["cost"])
 
+    # ---- per-k tuning picks (keyed by k and the graph's degree profile) and one batch
+    # mixing k = 3, 5 and 7 on the same graph, each job on its own k's pick
+    mixed = None
+    if not args.no_legs:
+        picks = {}
+        for kk in (3, 5, 7):
+            cand = [np.asarray(q, np.int32) for q in wl.queries[kk][-8:]]
+            picks[kk], _ = autotune.tune_k(eng, g, kk, cand, budget_s=20.0, batched=True)
+        log(f"# per-k picks (batched) {picks} ({time.time() - t0:.1f}s)")
+        mq = [engine.Query(np.asarray(q, np.int32)) for kk in (3, 5, 7) for q in wl.queries[kk][100:116]]
+        order = np.random.default_rng(datasets.SEED_BASE + 88).permutation(len(mq))
+        mq = [mq[i] for i in order]
+        eng.solve_batch(mq[:8])
+        tm0 = time.perf_counter()
+        mres = eng.solve_batch(mq)
+        mdt = time.perf_counter() - tm0
+        chk = mq[:6]
+        same = sum(1 for q, r in zip(chk, mres[:6]) if eng.solve(q.gids)["cost"] == r["cost"])
+        mixed = {"queries": len(mq), "k": [3, 5, 7], "qps": round(len(mq) / mdt, 3),
+                 "picks": {str(kk): v for kk, v in picks.items()}, "equal_plain": f"{same}/{len(chk)}"}
+        log(f"# mixed-k batch: {mixed}")
+
     # ---- CPU baseline: PrunedDP++ restatement on the host cores, bounded sample
     cpu = None
     parity = None
@@ -296,15 +494,27 @@ def main():
         import pyoracle
 
         inst = pyoracle.Instance(g, wl.groups)
-        cores = min(os.cpu_count() or 1, 16)
+        # one query per worker thread; the pool is sized to this GPU's share of the host
+        # (16 threads per GPU), whatever the host's core count
+        host_cpus = os.cpu_count() or 1
+        cores = min(host_cpus, 16)
         sample = np.stack(plain_q[:max(cores, 1)])
         tc = time.perf_counter()
         ores = pyoracle.batch(inst, sample, algo=0, n_threads=cores, time_budget_s=args.cpu_budget)
         cdt = time.perf_counter() - tc
         solved = [(i, o) for i, o in enumerate(ores) if o["status"] != -99]
-        cpu = {"value": round(len(solved) / cdt, 4), "unit": "queries/s", "cores": cores, "kind": "port",
+        e2e = np.array([o["t_total_s"] for _, o in solved]) * 1e3
+        srch = np.array([o["t_search_s"] for _, o in solved]) * 1e3
+        cpu = {"value": round(len(solved) / cdt, 4), "unit": "queries/s", "cores": cores, "host_cpus": host_cpus,
+               "kind": "port",
                "sample": f"{len(solved)} queries of k={args.k} on the same graph, one per thread, "
-                         f"{cdt:.1f}s wall"}
+                         f"{cdt:.1f}s wall",
+               # per query: end to end (SSSP, bounds, search, tree) and the reference's
+               # timing scope, the best-first loop only (CPUNONHOP.h:634-665)
+               "p50_ms": round(float(np.median(e2e)), 1) if len(e2e) else None,
+               "p99_ms": round(float(np.percentile(e2e, 99)), 1) if len(e2e) else None,
+               "search_p50_ms": round(float(np.median(srch)), 1) if len(srch) else None,
+               "search_p99_ms": round(float(np.percentile(srch, 99)), 1) if len(srch) else None}
         gpu_cost = {}
         for q, r in zip(plain_q, plain):
             gpu_cost[tuple(q.tolist())] = r["cost"]
@@ -315,6 +525,14 @@ def main():
         parity = f"{agree}/{len(checked)}"
         print(f"# cpu: {cpu['value']} q/s on {cores} threads; parity {parity} ({time.time() - t0:.1f}s)",
               file=sys.stderr, flush=True)
+    eng.close()
+    del eng
+
+    hop = served = None
+    if not args.no_legs:
+        hop = hop_leg(log)
+        served = served_leg(log)
+    traffic_b = traffic["bytes_per_round"] if traffic else None
 
     out = {
         "metric": "GST queries/s at k=5 (batched persistent path), p50/p99 ms, bit-exact vs PrunedDP++",
@@ -341,7 +559,10 @@ def main():
         "parity_vs_cpu": parity,
         "parity_batched_vs_plain": f"{plain_match}/{len(plain)}",
         "roofline": {"bound": "hbm", "achieved": round(grow_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(grow_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(grow_gbs / HBM_PEAK_GBS, 4), "traffic": traffic_b,
+                     # measured HBM bytes per grow round / algorithmic bytes per grow round
+                     "traffic_over_algorithmic": round(traffic_b / (gb / grow_rounds), 3) if traffic_b else None,
+                     "traffic_detail": traffic,
                      "kernel": "grow round: k_select + k_grow_thread + k_grow_wave (plain path, HIP events "
                                "on the engine stream)",
                      "launches": grow_rounds, "avg_launch_us": round(gt / grow_rounds * 1e6, 1),
@@ -350,9 +571,11 @@ def main():
                      # against bench.py --gather-ceiling
                      "arcs_g_per_s": round(arcs_gs, 2)},
         "cpu_baseline": cpu,
+        "mixed_k_batch": mixed,
+        "hop": hop,
+        "served": served,
     }
     print(json.dumps(out))
-    eng.close()
 
 
 if __name__ == "__main__":

@@ -1,6 +1,13 @@
-"""Autotuner of the grow phase: degree split (thread vs wavefront per vertex), edge chunk
-per wavefront item, and near-far step, chosen per (graph, k) by timing a sample of
-queries on the device.
+"""Autotuner of the grow and merge phases: degree split (thread vs wavefront per vertex),
+edge chunk per wavefront item, merge tile (vertices per merge work item) and near-far
+step, chosen by timing sample queries on the device.
+
+The reference fixes these shapes at compile time (THREAD_PER_BLOCK,
+code/TrimCDP/include/DPQ.cuh:43; SML_MID / MID_LRG and THDS_NUM / BLKS_NUM,
+code/TrimCDP-WB/include/util.h:19-20,32-33). Here they are picked per (k, degree
+profile): `tune_k` sets the tuning of the queries of k groups only (gst_set_tuning_k), so
+a batch that mixes k runs each job on its own pick, and keeps its picks keyed by the
+graph's degree profile and k, so a second engine on the same graph reuses them.
 
 The tuning changes only the schedule of the same exact DP, so every candidate returns
 the same optimum; the tuner checks that on its sample and refuses a candidate that
@@ -16,19 +23,44 @@ import numpy as np
 DEFAULT_GRID = {
     "degree_split": (16, 32, 64, 128, 256),
     "chunk_edges": (512, 1024, 4096),
-    "near_far_delta": (0, 64),
+    "near_far_delta": (0,),
 }
+
+K_GRID = {
+    "degree_split": (32, 64, 128),
+    "chunk_edges": (512, 1024, 4096),
+    "merge_tile": (0, 32),
+}
+
+# picks by (degree profile, k, diameter-bounded, batched)
+_PICKS: dict = {}
+
+
+def _time_queries(eng, queries, D, batched, repeats):
+    from . import engine
+
+    if batched:
+        batch = [engine.Query(np.asarray(q, np.int32), D) for q in queries]
+        eng.solve_batch(batch[:2])  # warm (graph capture, slot pool)
+        t = time.perf_counter()
+        for _ in range(repeats):
+            costs = [r["cost"] for r in eng.solve_batch(batch)]
+    else:
+        eng.solve(queries[0], D=D)  # warm (allocations, code paths)
+        t = time.perf_counter()
+        for _ in range(repeats):
+            costs = [eng.solve(q, D=D)["cost"] for q in queries]
+    return (time.perf_counter() - t) / (repeats * len(queries)), costs
 
 
 def tune(eng, queries, D=-1, grid=None, repeats=1, budget_s=60.0, log=None, batched=False):
-    """Time every grid point on `queries` and apply the fastest: one query at a time on
-    the plain path, or, with batched=True, all of them as one batch (the batched form
-    favours other settings: with many jobs in flight, lanes idle behind a long thread
-    item are covered by other waves, while per-item overheads are not).
+    """Time every grid point of (degree split, chunk, near-far step) on `queries` for all
+    k at once and apply the fastest: one query at a time on the plain path, or, with
+    batched=True, all of them as one batch (the batched form favours other settings: with
+    many jobs in flight, lanes idle behind a long thread item are covered by other waves,
+    while per-item overheads are not).
 
     Returns (best_config dict, table of (config, seconds per query))."""
-    from . import engine
-
     grid = grid or DEFAULT_GRID
     keys = list(grid)
     table = []
@@ -37,20 +69,8 @@ def tune(eng, queries, D=-1, grid=None, repeats=1, budget_s=60.0, log=None, batc
     best = None
     for vals in itertools.product(*(grid[k] for k in keys)):
         cfg = dict(zip(keys, vals))
-        eng.set_tuning(cfg["degree_split"], cfg["chunk_edges"], cfg["near_far_delta"])
-        if batched:
-            batch = [engine.Query(np.asarray(q, np.int32), D) for q in queries]
-            eng.solve_batch(batch[:2])  # warm (graph capture, slot pool)
-            t = time.perf_counter()
-            for _ in range(repeats):
-                costs = [r["cost"] for r in eng.solve_batch(batch)]
-        else:
-            eng.solve(queries[0], D=D)  # warm (allocations, code paths)
-            t = time.perf_counter()
-            costs = []
-            for _ in range(repeats):
-                costs = [eng.solve(q, D=D)["cost"] for q in queries]
-        dt = (time.perf_counter() - t) / (repeats * len(queries))
+        eng.set_tuning(cfg["degree_split"], cfg["chunk_edges"], cfg.get("near_far_delta", 0))
+        dt, costs = _time_queries(eng, queries, D, batched, repeats)
         if ref is None:
             ref = costs
         elif costs != ref:
@@ -63,8 +83,46 @@ def tune(eng, queries, D=-1, grid=None, repeats=1, budget_s=60.0, log=None, batc
         if time.perf_counter() - t_start > budget_s:
             break
     cfg = best[0]
-    eng.set_tuning(cfg["degree_split"], cfg["chunk_edges"], cfg["near_far_delta"])
+    eng.set_tuning(cfg["degree_split"], cfg["chunk_edges"], cfg.get("near_far_delta", 0))
     return cfg, table
+
+
+def tune_k(eng, graph, k, queries, D=-1, grid=None, repeats=1, budget_s=30.0, log=None, batched=False):
+    """Pick (degree split, chunk, merge tile) for the queries of k groups on this graph and
+    apply it to k only. The pick is cached under (degree profile, k, D >= 0, batched): a
+    later call with the same key applies it without timing anything.
+
+    Returns (config dict, table of (config, seconds per query); empty when cached)."""
+    key = (profile_key(graph), int(k), D >= 0, bool(batched))
+    if key in _PICKS:
+        cfg = _PICKS[key]
+        eng.set_tuning_k(k, cfg["degree_split"], cfg["chunk_edges"], cfg["merge_tile"])
+        return dict(cfg), []
+    grid = grid or K_GRID
+    keys = list(grid)
+    table = []
+    ref = None
+    best = None
+    t_start = time.perf_counter()
+    for vals in itertools.product(*(grid[x] for x in keys)):
+        cfg = dict(zip(keys, vals))
+        eng.set_tuning_k(k, cfg["degree_split"], cfg["chunk_edges"], cfg["merge_tile"])
+        dt, costs = _time_queries(eng, queries, D, batched, repeats)
+        if ref is None:
+            ref = costs
+        elif costs != ref:
+            raise RuntimeError(f"tuning {cfg} for k={k} changed the optimum: {costs} vs {ref}")
+        table.append((cfg, dt))
+        if log:
+            log(f"autotune k={k} {cfg} {dt * 1e3:.2f} ms/query")
+        if best is None or dt < best[1]:
+            best = (cfg, dt)
+        if time.perf_counter() - t_start > budget_s:
+            break
+    cfg = best[0]
+    eng.set_tuning_k(k, cfg["degree_split"], cfg["chunk_edges"], cfg["merge_tile"])
+    _PICKS[key] = dict(cfg)
+    return dict(cfg), table
 
 
 def degree_profile(graph):
@@ -72,3 +130,9 @@ def degree_profile(graph):
     d = graph.degrees()
     return {"n": int(graph.n), "m": int(graph.m), "p50": float(np.median(d)), "p99": float(np.percentile(d, 99)),
             "max": int(d.max())}
+
+
+def profile_key(graph):
+    """Hashable degree profile: graphs of the same size and degree quantiles share picks."""
+    p = degree_profile(graph)
+    return (p["n"], p["m"], round(p["p50"], 1), round(p["p99"], 1), p["max"])

@@ -30,6 +30,8 @@ struct CohortConst {
   int32_t *allitems;         // [icap][2] items of every vertex (the targets of a pull round)
   unsigned long long *stx;   // [NSTRIPE][NSTX] cohort work counters
   long long icap;
+  const int32_t *nchunk;     // the members' grow tuning (one k, so one tuning)
+  int32_t chunk, n_heavy;
   int32_t B, K, L, C, S, NW; // members, k, level, labels per member, row words, chunks
   int32_t jobs[CMAXB];       // member index -> job index in the cohort's job table
 };
@@ -81,7 +83,7 @@ __global__ __launch_bounds__(BLK) void k_ccollect(const CohortConst *__restrict_
     int cnt = 0, nch = 0;
     if (v < g.n && flag[v]) {
       flag[v] = 0;
-      nch = chunks_of(g, (int32_t)v);
+      nch = chunks_of(cc.nchunk, cc.n_heavy, (int32_t)v);
       cnt = nch ? nch : 1;
     }
     const int pos = wave_append(&cs->cnt[nx], cnt);
@@ -102,7 +104,7 @@ __global__ __launch_bounds__(BLK) void k_callitems(const CohortConst *__restrict
   const long long stride = (long long)gridDim.x * BLK;
   for (long long b = (long long)blockIdx.x * BLK + (threadIdx.x & ~63); b < g.n; b += stride) {
     const long long v = b + lane_id();
-    const int nch = v < g.n ? chunks_of(g, (int32_t)v) : 0;
+    const int nch = v < g.n ? chunks_of(cc.nchunk, cc.n_heavy, (int32_t)v) : 0;
     const int cnt = v < g.n ? (nch ? nch : 1) : 0;
     const int pos = wave_append(&cs->nall, cnt);
     if (cnt) {
@@ -208,8 +210,8 @@ __global__ __launch_bounds__(BLK) void k_cgrow(const CohortConst *__restrict__ c
     }
     if (!nb) continue;
     const int32_t vb = g.beg[v], ve = g.beg[v + 1];
-    const int32_t eb = ch < 0 ? vb : vb + ch * g.chunk;
-    const int32_t ee = ch < 0 ? ve : min(ve, eb + g.chunk);
+    const int32_t eb = ch < 0 ? vb : vb + ch * cc.chunk;
+    const int32_t ee = ch < 0 ? ve : min(ve, eb + cc.chunk);
     n_edges += ee - eb;
     n_relax += (unsigned long long)(ee - eb) * nb;
     for (int32_t e0 = eb; e0 < ee; e0 += WAVE) {
@@ -297,8 +299,8 @@ __global__ __launch_bounds__(BLK) void k_cpull(const CohortConst *__restrict__ c
     const int32_t u = __builtin_amdgcn_readfirstlane(itm[2 * it]);
     const int32_t ch = __builtin_amdgcn_readfirstlane(itm[2 * it + 1]);
     const int32_t ub_ = g.beg[u], ue = g.beg[u + 1];
-    const int32_t eb = ch < 0 ? ub_ : ub_ + ch * g.chunk;
-    const int32_t ee = ch < 0 ? ue : min(ue, eb + g.chunk);
+    const int32_t eb = ch < 0 ? ub_ : ub_ + ch * cc.chunk;
+    const int32_t ee = ch < 0 ? ue : min(ue, eb + cc.chunk);
     int32_t acc[CMAXCH];
 #pragma unroll
     for (int c = 0; c < CMAXCH; c++) acc[c] = INF;

@@ -89,7 +89,15 @@ struct gst_ctx {
   int64_t m = 0;
   GraphDev g{};
   int32_t *d_beg = nullptr, *d_adj = nullptr, *d_w = nullptr, *d_gverts = nullptr;
-  int32_t *d_nchunk = nullptr;  // per vertex: wavefront chunks (0 = thread item)
+  // grow tunings: chunk counts per vertex for each (degree split, chunk) in use
+  struct TuneSet {
+    int32_t split = 0, chunk = 0, n_heavy = 0;
+    int32_t *d_nchunk = nullptr;  // per vertex: wavefront chunks (0 = thread item)
+    long long items = 0;          // wavefront items of all heavy vertices
+  };
+  std::vector<TuneSet> tunes;
+  int tune_k[MAXK + 1] = {0};     // per k: index into tunes
+  int tile_k[MAXK + 1] = {0};     // per k: merge tile cap in vertices (0 = as many as LDS holds)
   long long *d_goffs = nullptr;
   std::vector<int64_t> h_goffs;
   std::vector<int32_t> h_gverts;    // group members, caller's ids
@@ -100,7 +108,7 @@ struct gst_ctx {
   int64_t graph_bytes = 0;
   long long bcap = 0;  // wavefront items one worklist can hold
   gst_opts opts{};
-  int32_t split = 32, chunk = 1024, delta = 0;
+  int32_t delta = 0;
   // single-query path: one slot per (K, D) shape, least recently used evicted beyond
   // plain_cap bytes (a mixed stream of shapes would otherwise reallocate its arena,
   // up to V * 2^k * (D+1) * 4 bytes, on almost every query)
@@ -238,6 +246,13 @@ void fill_jobconst(const gst_ctx *c, const Slot &s, const Query &q, JobConst &jc
   jc.K = q.K; jc.D = q.D; jc.H = q.D >= 0 ? q.D + 1 : 1; jc.W = 1 << q.K; jc.R = jc.W - 1;
   jc.hop = q.D >= 0; jc.flags = q.flags; jc.nl = s.nlst;
   jc.delta = q.D >= 0 ? 0 : c->delta;
+  {
+    const gst_ctx::TuneSet &ts = c->tunes[c->tune_k[q.K]];
+    jc.nchunk = ts.d_nchunk;
+    jc.split = ts.split;
+    jc.chunk = ts.chunk;
+    jc.n_heavy = ts.n_heavy;
+  }
   for (int i = 0; i < q.K; i++) jc.gid[i] = q.gids[i];
   long long members = 0;
   for (int i = 0; i < q.K; i++) members += c->h_goffs[q.gids[i] + 1] - c->h_goffs[q.gids[i]];
@@ -250,6 +265,7 @@ void fill_jobconst(const gst_ctx *c, const Slot &s, const Query &q, JobConst &jc
   jc.tv = jc.hop ? std::max(1, std::min({256, HMERGE_LDS_WORDS / (jc.H * ws), HMERGE_MASKS / jc.H,
                                          SUH * BLK / (jc.H * cmax)}))
                  : std::min(512, MERGE_LDS_WORDS / ws);
+  if (c->tile_k[q.K] > 0) jc.tv = std::min(jc.tv, c->tile_k[q.K]);
   int cnt[8] = {0};
   for (int p = 0; p < jc.W; p++) {
     const int L = __builtin_popcount(p);
@@ -584,8 +600,8 @@ int gst_open(const gst_csr *csr, const gst_groups *groups, const gst_opts *opts,
   c->n = n;
   c->m = m;
   c->n_groups = groups->n_groups;
-  c->split = c->opts.degree_split > 0 ? c->opts.degree_split : 32;
-  c->chunk = c->opts.chunk_edges > 0 ? ((c->opts.chunk_edges + 63) / 64) * 64 : 1024;
+  const int32_t split0 = c->opts.degree_split > 0 ? c->opts.degree_split : 32;
+  const int32_t chunk0 = c->opts.chunk_edges > 0 ? ((c->opts.chunk_edges + 63) / 64) * 64 : 1024;
   c->delta = c->opts.near_far_delta > 0 ? c->opts.near_far_delta : 0;
   c->max_batch = c->opts.max_batch > 0 ? c->opts.max_batch : 64;
   DeviceGraph dg = renumber(csr, groups, !(c->opts.flags & GST_O_KEEP_ORDER));
@@ -602,7 +618,6 @@ int gst_open(const gst_csr *csr, const gst_groups *groups, const gst_opts *opts,
   HIPCHK(hipMalloc((void **)&c->d_w, sizeof(int32_t) * std::max<int64_t>(m, 1)));
   HIPCHK(hipMalloc((void **)&c->d_goffs, sizeof(long long) * c->h_goffs.size()));
   HIPCHK(hipMalloc((void **)&c->d_gverts, sizeof(int32_t) * std::max<int64_t>(total, 1)));
-  HIPCHK(hipMalloc((void **)&c->d_nchunk, sizeof(int32_t) * n));
   HIPCHK(hipMemcpy(c->d_beg, beg32.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice));
   if (m) {
     HIPCHK(hipMemcpy(c->d_adj, dg.adj ? dg.adj : dg.adj_v.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice));
@@ -626,13 +641,13 @@ int gst_open(const gst_csr *csr, const gst_groups *groups, const gst_opts *opts,
   HIPCHK(hipEventCreate(&c->ev0));
   HIPCHK(hipEventCreate(&c->ev1));
   c->g.beg = c->d_beg; c->g.adj = c->d_adj; c->g.w = c->d_w;
-  c->g.goffs = c->d_goffs; c->g.gverts = c->d_gverts; c->g.nchunk = c->d_nchunk; c->g.n = n;
+  c->g.goffs = c->d_goffs; c->g.gverts = c->d_gverts; c->g.n = n;
   {
     size_t fr = 0, tot = 0;
     HIPCHK(hipMemGetInfo(&fr, &tot));
     c->plain_cap = std::min<int64_t>((int64_t)fr / 8, (int64_t)32 << 30);
   }
-  int rc = gst_set_tuning(c.get(), c->split, c->chunk, c->delta);
+  int rc = gst_set_tuning(c.get(), split0, chunk0, c->delta);
   if (rc) return rc;
   rc = batch_open(c.get());
   if (rc) return rc;
@@ -642,45 +657,94 @@ int gst_open(const gst_csr *csr, const gst_groups *groups, const gst_opts *opts,
   return GST_OK;
 }
 
-int gst_set_tuning(gst_ctx *c, int32_t degree_split, int32_t chunk_edges, int32_t near_far_delta) {
-  // a wavefront of thread items sums 64 degrees below the split in 32 bits
-  if (!c || degree_split < 1 || degree_split > (1 << 24) || chunk_edges < 64 || chunk_edges > (1 << 24) ||
-      near_far_delta < 0) {
-    set_err("bad tuning (need 1 <= degree_split <= 2^24, 64 <= chunk_edges <= 2^24, near_far_delta >= 0)");
-    return GST_BAD_INPUT;
+}  // extern "C"
+
+namespace {
+
+// Index of the tuning (split, chunk) in c->tunes, building its chunk counts when new.
+// A new tuning whose wavefront items exceed the worklist capacity rebuilds the slots.
+int tune_index(gst_ctx *c, int32_t split, int32_t chunk, int &idx) {
+  for (size_t i = 0; i < c->tunes.size(); i++)
+    if (c->tunes[i].split == split && c->tunes[i].chunk == chunk) { idx = (int)i; return GST_OK; }
+  // drop tunings no k uses any more (each holds n chunk counts on the device)
+  for (size_t i = c->tunes.size(); i-- > 0;) {
+    bool used = false;
+    for (int k = 0; k <= MAXK; k++) used = used || c->tune_k[k] == (int)i;
+    if (used) continue;
+    (void)hipFree(c->tunes[i].d_nchunk);
+    c->tunes.erase(c->tunes.begin() + i);
+    for (int k = 0; k <= MAXK; k++)
+      if (c->tune_k[k] > (int)i) c->tune_k[k]--;
   }
-  c->delta = near_far_delta;
-  chunk_edges = ((chunk_edges + 63) / 64) * 64;
-  long long bcap = 0;
+  gst_ctx::TuneSet ts;
+  ts.split = split;
+  ts.chunk = chunk;
   std::vector<int32_t> nch(c->n, 0);
   for (int32_t v = 0; v < c->n; v++)
-    if (c->h_deg[v] >= degree_split) {
-      nch[v] = (c->h_deg[v] + chunk_edges - 1) / chunk_edges;
-      bcap += nch[v];
+    if (c->h_deg[v] >= split) {
+      nch[v] = (c->h_deg[v] + chunk - 1) / chunk;
+      ts.items += nch[v];
     }
-  HIPCHK(hipMemcpy(c->d_nchunk, nch.data(), sizeof(int32_t) * c->n, hipMemcpyHostToDevice));
-  bcap = std::max<long long>(bcap, 1);
+  HIPCHK(hipMalloc((void **)&ts.d_nchunk, sizeof(int32_t) * std::max<int32_t>(c->n, 1)));
+  HIPCHK(hipMemcpy(ts.d_nchunk, nch.data(), sizeof(int32_t) * c->n, hipMemcpyHostToDevice));
+  // with vertices renumbered by descending degree the wavefront vertices are a prefix
+  ts.n_heavy = c->n;
+  if (!c->old_of_new.empty()) {
+    int32_t h = 0;
+    while (h < c->n && c->h_deg[h] >= split) h++;
+    bool prefix = true;
+    for (int32_t v = h; v < c->n && prefix; v++) prefix = c->h_deg[v] < split;
+    if (prefix) ts.n_heavy = h;
+  }
+  const long long bcap = std::max<long long>(ts.items, 1);
   if (bcap > c->bcap) {
     // slots sized for the old capacity must be rebuilt
     for (auto &e : c->plain) slot_free(e.second);
     c->plain.clear();
     batch_release_slots(c);
+    c->bcap = bcap;
   }
-  c->bcap = std::max(c->bcap, bcap);
-  c->split = degree_split;
-  c->chunk = chunk_edges;
-  c->g.split = degree_split;
-  c->g.chunk = chunk_edges;
-  // with vertices renumbered by descending degree the wavefront vertices are a prefix
-  c->g.n_heavy = c->n;
-  if (!c->old_of_new.empty()) {
-    int32_t h = 0;
-    while (h < c->n && c->h_deg[h] >= degree_split) h++;
-    bool prefix = true;
-    for (int32_t v = h; v < c->n && prefix; v++) prefix = c->h_deg[v] < degree_split;
-    if (prefix) c->g.n_heavy = h;
+  c->tunes.push_back(ts);
+  idx = (int)c->tunes.size() - 1;
+  return GST_OK;
+}
+
+int check_tuning(int32_t degree_split, int32_t chunk_edges) {
+  // a wavefront of thread items sums 64 degrees below the split in 32 bits
+  if (degree_split < 1 || degree_split > (1 << 24) || chunk_edges < 64 || chunk_edges > (1 << 24)) {
+    set_err("bad tuning (need 1 <= degree_split <= 2^24, 64 <= chunk_edges <= 2^24)");
+    return GST_BAD_INPUT;
   }
-  batch_drop_graph(c);
+  return GST_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gst_set_tuning(gst_ctx *c, int32_t degree_split, int32_t chunk_edges, int32_t near_far_delta) {
+  if (!c || near_far_delta < 0) { set_err("bad tuning (need near_far_delta >= 0)"); return GST_BAD_INPUT; }
+  if (int rc = check_tuning(degree_split, chunk_edges)) return rc;
+  chunk_edges = ((chunk_edges + 63) / 64) * 64;
+  for (int k = 0; k <= MAXK; k++) c->tune_k[k] = -1;  // every tuning becomes unused ...
+  int idx = 0;
+  if (int rc = tune_index(c, degree_split, chunk_edges, idx)) return rc;
+  for (int k = 0; k <= MAXK; k++) { c->tune_k[k] = idx; c->tile_k[k] = 0; }  // ... but this one
+  c->delta = near_far_delta;
+  return GST_OK;
+}
+
+int gst_set_tuning_k(gst_ctx *c, int32_t k, int32_t degree_split, int32_t chunk_edges, int32_t merge_tile) {
+  if (!c || k < 1 || k > MAXK || merge_tile < 0) { set_err("bad tuning (need 1 <= k <= GST_MAX_K, merge_tile >= 0)"); return GST_BAD_INPUT; }
+  if (int rc = check_tuning(degree_split, chunk_edges)) return rc;
+  chunk_edges = ((chunk_edges + 63) / 64) * 64;
+  const int old = c->tune_k[k];
+  c->tune_k[k] = -1;
+  int idx = 0;
+  const int rc = tune_index(c, degree_split, chunk_edges, idx);
+  if (rc) { c->tune_k[k] = old; return rc; }
+  c->tune_k[k] = idx;
+  c->tile_k[k] = merge_tile;
   return GST_OK;
 }
 
@@ -854,7 +918,8 @@ void gst_close(gst_ctx *c) {
   for (auto &e : c->plain) slot_free(e.second);
   c->plain.clear();
   (void)hipFree(c->d_beg); (void)hipFree(c->d_adj); (void)hipFree(c->d_w);
-  (void)hipFree(c->d_goffs); (void)hipFree(c->d_gverts); (void)hipFree(c->d_nchunk);
+  (void)hipFree(c->d_goffs); (void)hipFree(c->d_gverts);
+  for (auto &t : c->tunes) (void)hipFree(t.d_nchunk);
   (void)hipFree(c->d_jc); (void)hipFree(c->d_js); (void)hipHostFree(c->h_js);
   (void)hipHostFree(c->h_jc); (void)hipHostFree(c->h_stx); (void)hipHostFree(c->h_count);
   (void)hipFree(c->d_count);

@@ -93,6 +93,11 @@ struct JobConst {
   int32_t tv;                 // vertices per merge tile
   int32_t delta;              // near-far step of the grow frontier (0 = off)
   int32_t passive;            // cohort member: the grow phases run in the cohort kernels
+  // grow tuning of this job (chosen per k): vertices of degree >= split are grown as
+  // ceil(deg / chunk) wavefront items (nchunk[v]; 0 = thread item), the others by threads;
+  // with vertices numbered by descending degree those are the ids below n_heavy
+  const int32_t *nchunk;
+  int32_t split, chunk, n_heavy;
   // cohort member only: the merge of a level L < K writes its states into the cohort's
   // level block cT (row of vertex v at v * S, S = cohort_stride(cB, C(k,L)), this member's
   // labels at cq * C(k,L)) and activates them in list 1 of the cohort's label masks
@@ -143,11 +148,7 @@ struct GraphDev {
   const int32_t *w;     // [m]
   const long long *goffs;
   const int32_t *gverts;
-  const int32_t *nchunk; // [n]: 0 if deg < split (thread item), else ceil(deg / chunk)
   int32_t n;
-  int32_t split;        // degree split thread/wavefront
-  int32_t chunk;        // edges per wavefront item
-  int32_t n_heavy;      // vertices >= n_heavy have deg < split (nchunk 0); n when unknown
 };
 
 // Costs saturate at INF: a grow step c + w or a merge c1 + c2 at or above INF is stored
@@ -164,8 +165,11 @@ __device__ __forceinline__ void flag_saturation(JobState *s) {
 
 // nchunk[u] without a memory access for light vertices: with vertices numbered by
 // descending degree, those of degree >= split are exactly the ids below n_heavy.
-__device__ __forceinline__ int32_t chunks_of(const GraphDev &g, int32_t u) {
-  return u < g.n_heavy ? g.nchunk[u] : 0;
+__device__ __forceinline__ int32_t chunks_of(const int32_t *nchunk, int32_t n_heavy, int32_t u) {
+  return u < n_heavy ? nchunk[u] : 0;
+}
+__device__ __forceinline__ int32_t chunks_of(const JobConst &c, int32_t u) {
+  return chunks_of(c.nchunk, c.n_heavy, u);
 }
 
 constexpr int JF_NO_PRUNE = 4;  // == GST_F_NO_PRUNE
@@ -228,7 +232,7 @@ __device__ __forceinline__ void push_item(const JobConst &c, JobState *s, const 
 __device__ __forceinline__ void push_vertex(const JobConst &c, JobState *s, const GraphDev &g,
                                             int li, int32_t u) {
   const int32_t d = g.beg[u + 1] - g.beg[u];
-  push_item(c, s, g, li, u, d < g.split ? 0 : (d + g.chunk - 1) / g.chunk);
+  push_item(c, s, g, li, u, d < c.split ? 0 : (d + c.chunk - 1) / c.chunk);
 }
 
 // Admissible lower bound on the weight still needed to connect a tree rooted at v
@@ -911,7 +915,7 @@ __global__ __launch_bounds__(BLK) void k_farsplit(const JobConst *jcs, JobState 
       far[v] = 0;
       lo = s->parity ^ 1;
       fl = FAR_LIST + (s->fpar ^ 1);
-      nch = g.nchunk[v];
+      nch = chunks_of(c, v);
       unsigned long long defer = 0;
       int dummy = 0;
       const unsigned long long near = grow_select(c, s, v, a, s->level, load_ub(s), dummy,
@@ -935,7 +939,7 @@ __global__ __launch_bounds__(BLK) void k_farsplit(const JobConst *jcs, JobState 
 // skipped (want[j] = false, nch[j] = 0).
 template <int NE>
 __device__ __forceinline__ void relax_edges(int32_t *dbase, size_t dstride, unsigned long long *act_lo,
-                                            const GraphDev &g, const RelaxGroup &G, const bool (&h)[NE],
+                                            const int32_t *nchk, int32_t nheavy, const RelaxGroup &G, const bool (&h)[NE],
                                             const int32_t (&u)[NE], const int32_t (&w)[NE], bool (&want)[NE],
                                             int (&nch)[NE], int &n_atom) {
   gi32 *Td[NE];
@@ -975,7 +979,7 @@ __device__ __forceinline__ void relax_edges(int32_t *dbase, size_t dstride, unsi
                   ? __hip_atomic_fetch_or(&act[u[j]], imp[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0
                   : false;
 #pragma unroll
-  for (int j = 0; j < NE; j++) nch[j] = want[j] ? chunks_of(g, u[j]) : 0;
+  for (int j = 0; j < NE; j++) nch[j] = want[j] ? chunks_of(nchk, nheavy, u[j]) : 0;
 }
 
 // Thread items (degree < split), edge-balanced. A wavefront takes 64 consecutive items and
@@ -994,6 +998,8 @@ struct ThreadSeg {
   int32_t *dbase[WAVE];
   unsigned long long *act[WAVE];
   long long dstride[WAVE];
+  const int32_t *nck[WAVE];  // the item's job: chunk counts and n_heavy
+  int32_t nhv[WAVE];
 };
 
 __global__ __launch_bounds__(BLK) void k_grow_thread(const JobConst *__restrict__ jcs, JobState *jss, const Plan *pl, GraphDev g) {
@@ -1016,9 +1022,9 @@ __global__ __launch_bounds__(BLK) void k_grow_thread(const JobConst *__restrict_
   // Fields of job ucj, wave-uniform: most waves hold items of one job only, and these
   // are read once per job instead of once per pass (each read of the job state would be
   // a dependent round trip ahead of the items' loads).
-  int ucj = -1, ulo = 0;
+  int ucj = -1, ulo = 0, uheavy = 0;
   long long urow = 0;
-  const int32_t *ulst = nullptr, *usrc = nullptr;
+  const int32_t *ulst = nullptr, *usrc = nullptr, *unck = nullptr;
   int32_t *udst = nullptr;
   const unsigned long long *uacur = nullptr;
   unsigned long long *uanxt = nullptr;
@@ -1044,6 +1050,8 @@ __global__ __launch_bounds__(BLK) void k_grow_thread(const JobConst *__restrict_
       uanxt = c.act + (size_t)ulo * g.n;
       usrc = lrow(c, L, 0, d);
       udst = lrow(c, L, 0, c.hop ? d + 1 : 0);
+      unck = c.nchunk;
+      uheavy = c.n_heavy;
     }
     if (it < total) {
       int jb, lo;
@@ -1057,6 +1065,8 @@ __global__ __launch_bounds__(BLK) void k_grow_thread(const JobConst *__restrict_
         S.dbase[lane] = udst;
         S.dstride[lane] = urow;
         S.act[lane] = uanxt;
+        S.nck[lane] = unck;
+        S.nhv[lane] = uheavy;
       } else {  // a wave across two jobs: per-lane fields
         jb = find_job(soff, nj, it);
         const JobConst &c = jcs[jb];
@@ -1070,6 +1080,8 @@ __global__ __launch_bounds__(BLK) void k_grow_thread(const JobConst *__restrict_
         S.dbase[lane] = lrow(c, L, 0, c.hop ? d + 1 : 0);
         S.dstride[lane] = (long long)c.H * c.nsub[L];
         S.act[lane] = c.act + (size_t)lo * g.n;
+        S.nck[lane] = c.nchunk;
+        S.nhv[lane] = c.n_heavy;
       }
       if (jb != cj) { flush_lane(jcs, jss, cj, k); k.zero(); cj = jb; }
       k.verts++;
@@ -1125,7 +1137,7 @@ __global__ __launch_bounds__(BLK) void k_grow_thread(const JobConst *__restrict_
           Gj.n = S.gn[j];
 #pragma unroll
           for (int i = 0; i < RG; i++) { Gj.p[i] = S.gp[j][i]; Gj.c[i] = S.gc[j][i]; }
-          relax_edges<1>(S.dbase[j], (size_t)S.dstride[j], S.act[j], g, Gj, h, u, w, want, nch, k.atom);
+          relax_edges<1>(S.dbase[j], (size_t)S.dstride[j], S.act[j], S.nck[j], S.nhv[j], Gj, h, u, w, want, nch, k.atom);
           ljb = S.jb[j];
           llo = S.lo[j];
         }
@@ -1161,9 +1173,10 @@ __global__ __launch_bounds__(BLK) void k_grow_wave(const JobConst *__restrict__ 
   const long long waves_per_grid = (long long)gridDim.x * (BLK / WAVE);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: scalar registers
   // fields of job cj (set when cj changes)
-  int jlo = 0, jL = 0, jd = 0;
+  int jlo = 0, jL = 0, jd = 0, jheavy = 0;
   long long jrow = 0;
   int32_t *jsrc = nullptr, *jdst = nullptr;
+  const int32_t *jnck = nullptr;
   unsigned long long *jact = nullptr;
   // The wave's items are it0, it0 + S, it0 + 2S, ... (S = waves in the grid). Each lane
   // fetches the header of one of the next 64 (vertex, chunk, active mask, arc range, job),
@@ -1180,8 +1193,8 @@ __global__ __launch_bounds__(BLK) void k_grow_wave(const JobConst *__restrict__ 
       hv = item[0];
       hch = item[1];
       hgm = as_global(c.act)[(size_t)li * g.n + hv];  // already reduced by k_select
-      heb = g.beg[hv] + hch * g.chunk;
-      hee = min(g.beg[hv + 1], heb + g.chunk);
+      heb = g.beg[hv] + hch * c.chunk;
+      hee = min(g.beg[hv + 1], heb + c.chunk);
     }
     const long long left = (total - base + waves_per_grid - 1) / waves_per_grid;
     const int nitems = (int)(left < WAVE ? left : WAVE);
@@ -1202,6 +1215,8 @@ __global__ __launch_bounds__(BLK) void k_grow_wave(const JobConst *__restrict__ 
       jsrc = lrow(c, jL, 0, jd);
       jdst = lrow(c, jL, 0, c.hop ? jd + 1 : 0);
       jact = c.act + (size_t)jlo * g.n;
+      jnck = c.nchunk;
+      jheavy = __builtin_amdgcn_readfirstlane(c.n_heavy);
     }
     const int lo = jlo;
     const size_t dstride = (size_t)jrow;
@@ -1235,7 +1250,7 @@ __global__ __launch_bounds__(BLK) void k_grow_wave(const JobConst *__restrict__ 
           u[j] = h[j] ? adj[ej] : 0;
           w[j] = h[j] ? wts[ej] : 0;
         }
-        relax_edges<GW_NE>(dbase, dstride, act_lo, g, G, h, u, w, want, nch, k.atom);
+        relax_edges<GW_NE>(dbase, dstride, act_lo, jnck, jheavy, G, h, u, w, want, nch, k.atom);
 #pragma unroll
         for (int j = 0; j < GW_NE; j++) {
           wp_push(pb, jcs, jss, g, jb, lo, want[j], u[j], nch[j]);
@@ -1773,7 +1788,7 @@ __global__ __launch_bounds__(BLK) void k_collect(const JobConst *__restrict__ jc
     const int li = c.hop ? (int)(r / nblk) : (s->parity ^ 1);
     const long long v = (c.hop ? r % nblk : r) * 64 + lane_id();
     const bool want = v < g.n && c.act[(size_t)li * g.n + v] != 0;
-    const int nch = want ? chunks_of(g, (int32_t)v) : 0;
+    const int nch = want ? chunks_of(c, (int32_t)v) : 0;
     wp_push(pb, jcs, jss, g, jb, li, want, (int32_t)v, nch);
   }
   wp_flush(pb, jcs, jss, g);

@@ -40,7 +40,8 @@ class GstGroups(ctypes.Structure):
 class GstOpts(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int32), ("degree_split", ctypes.c_int32), ("chunk_edges", ctypes.c_int32),
                 ("max_batch", ctypes.c_int32), ("dp_budget_bytes", ctypes.c_int64), ("flags", ctypes.c_int32),
-                ("graph_rounds", ctypes.c_int32), ("near_far_delta", ctypes.c_int32), ("cohort_max", ctypes.c_int32)]
+                ("graph_rounds", ctypes.c_int32), ("near_far_delta", ctypes.c_int32), ("cohort_max", ctypes.c_int32),
+                ("cohort_min", ctypes.c_int32)]
 
 
 class GstQuery(ctypes.Structure):
@@ -69,7 +70,7 @@ class GstInfo(ctypes.Structure):
 
 EXPORTS = ["gst_open", "gst_solve", "gst_solve_batch", "gst_submit", "gst_poll", "gst_wait", "gst_info_get",
            "gst_set_tuning", "gst_result_free", "gst_close", "gst_strerror", "gst_last_error", "gst_arena_bytes",
-           "gst_job_bytes", "gst_solve_noalloc"]
+           "gst_job_bytes", "gst_solve_noalloc", "gst_set_tuning_k"]
 
 
 def lib_path() -> str:
@@ -94,6 +95,7 @@ def lib():
         L.gst_wait.argtypes = [P, ctypes.c_uint64, ctypes.POINTER(GstResult)]
         L.gst_info_get.argtypes = [P, ctypes.POINTER(GstInfo)]
         L.gst_set_tuning.argtypes = [P, i32, i32, i32]
+        L.gst_set_tuning_k.argtypes = [P, i32, i32, i32, i32]
         L.gst_result_free.argtypes = [ctypes.POINTER(GstResult)]
         L.gst_close.argtypes = [P]
         L.gst_strerror.restype = ctypes.c_char_p
@@ -135,7 +137,7 @@ class Engine:
     """One resident graph + group table on one GPU."""
 
     def __init__(self, graph, groups, device=0, degree_split=0, chunk_edges=0, max_batch=0,
-                 dp_budget_bytes=0, flags=0, graph_rounds=0, near_far_delta=0, cohort_max=0):
+                 dp_budget_bytes=0, flags=0, graph_rounds=0, near_far_delta=0, cohort_max=0, cohort_min=0):
         L = lib()
         self._beg = np.ascontiguousarray(graph.beg_pos, np.int64)
         self._adj = np.ascontiguousarray(graph.adj, np.int32)
@@ -146,7 +148,7 @@ class Engine:
                      self._w.ctypes.data)
         grp = GstGroups(len(self._goffs) - 1, self._goffs.ctypes.data, self._gverts.ctypes.data)
         opts = GstOpts(device, degree_split, chunk_edges, max_batch, dp_budget_bytes, flags, graph_rounds,
-                       near_far_delta, cohort_max)
+                       near_far_delta, cohort_max, cohort_min)
         h = ctypes.c_void_p()
         rc = L.gst_open(ctypes.byref(csr), ctypes.byref(grp), ctypes.byref(opts), ctypes.byref(h))
         if rc != GST_OK:
@@ -217,6 +219,12 @@ class Engine:
 
     def set_tuning(self, degree_split, chunk_edges, near_far_delta=0):
         rc = lib().gst_set_tuning(self._h, int(degree_split), int(chunk_edges), int(near_far_delta))
+        if rc:
+            raise GstError(rc, lib().gst_last_error().decode())
+
+    def set_tuning_k(self, k, degree_split, chunk_edges, merge_tile=0):
+        """Grow tuning and merge tile for queries of k groups only."""
+        rc = lib().gst_set_tuning_k(self._h, int(k), int(degree_split), int(chunk_edges), int(merge_tile))
         if rc:
             raise GstError(rc, lib().gst_last_error().decode())
 

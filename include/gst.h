@@ -107,8 +107,10 @@ typedef struct {
                               than the threshold wait until the near frontier drains
                               (<= 0 = off: plain label-correcting rounds) */
   int32_t cohort_max;      /* batched/served forms: queued non-hop queries of one k are
-                              grown together, up to this many at a time (0 = up to 64, as
-                              many as fit; 1 = each query on its own) */
+                              grown together through level 1, up to this many at a time
+                              (0 = up to 64, as many as fit; 1 = each query on its own) */
+  int32_t cohort_min;      /* fewest queued queries such a cohort is formed with (0 = 32;
+                              smaller cohorts hold up the other jobs for less gain) */
 } gst_opts;
 
 typedef struct {
@@ -174,6 +176,12 @@ int gst_info_get(gst_ctx *ctx, gst_info *info);
  * near-far step, 0 = off); used by the autotuner. Call while no batched or served query
  * is in flight. */
 int gst_set_tuning(gst_ctx *ctx, int32_t degree_split, int32_t chunk_edges, int32_t near_far_delta);
+/* The same per k (queries of k groups after de-duplication), plus the merge tile: at most
+ * merge_tile vertices per merge work item (0 = as many as fit the LDS). Queries of other k
+ * keep their tuning; a batch mixing k runs each job with its own. Used by the autotuner,
+ * which keys its picks by (k, degree profile). Call while no batched or served query is in
+ * flight. */
+int gst_set_tuning_k(gst_ctx *ctx, int32_t k, int32_t degree_split, int32_t chunk_edges, int32_t merge_tile);
 void gst_result_free(gst_result *r);
 void gst_close(gst_ctx *ctx);
 const char *gst_strerror(int code);

@@ -170,3 +170,20 @@ def test_serving_rejects_bad_and_oversized(rmat):
         r1 = eng.wait(t[1])
         assert r0["status"] == engine.GST_OVER_BUDGET
         assert r1["status"] == engine.GST_BAD_INPUT
+
+
+def test_per_k_tuning_in_mixed_batch(rmat, rmat_inst):
+    """Each k on its own grow tuning and merge tile (gst_set_tuning_k): a batch mixing k,
+    where every job runs with its k's settings, equals the oracle and the plain path."""
+    qs = _mixed_queries(rmat, 30, 21)
+    with engine.Engine(rmat.graph, rmat.groups, device=0) as eng:
+        eng.set_tuning_k(3, 4, 64, 8)        # hubs as many small wavefront items, tiny tiles
+        eng.set_tuning_k(5, 256, 4096, 0)    # nearly every vertex a thread item
+        eng.set_tuning_k(7, 48, 128, 3)
+        print("per-k tuning batch of", len(qs), flush=True)
+        res = eng.solve_batch(qs)
+        for q, r in zip(qs, res):
+            alone = eng.solve(q.gids, D=q.D)
+            _assert_same(rmat, rmat_inst, q, r, alone)
+        with pytest.raises(engine.GstError):
+            eng.set_tuning_k(8, 32, 1024, 0)

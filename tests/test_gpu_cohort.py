@@ -1,7 +1,8 @@
 """Cohorts: queued non-hop queries of one k grown together over a member-interleaved level
-block (the batched and served forms). Every member's result equals the same query solved
-alone and the CPU oracle; trees are valid; the finite-state count pins that every state
-of every member was written."""
+block (the batched and served forms), then handed to the per-query jobs. Every member's
+result equals the same query solved alone and the CPU oracle; trees are valid; the
+finite-state count pins that every state of every member was written. The engine here
+forms cohorts from two queries on (the default waits for 32)."""
 import numpy as np
 import pytest
 
@@ -9,6 +10,13 @@ import pyoracle
 from gpu4gst_1_amd import engine
 
 pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ceng(rmat):
+    eng = engine.Engine(rmat.graph, rmat.groups, device=0, cohort_min=2)
+    yield eng
+    eng.close()
 
 
 def _queries(rmat, ks, per_k, seed, D=-1):
@@ -29,15 +37,15 @@ def _check(rmat, inst, q, r, alone):
     assert ok, (q.gids, why)
 
 
-def test_cohort_k5(gpu_engine, rmat, rmat_inst):
+def test_cohort_k5(ceng, rmat, rmat_inst):
     qs = _queries(rmat, [5], 24, 2024)
     print("cohort of", len(qs), "k=5 queries", flush=True)
-    res = gpu_engine.solve_batch(qs)
+    res = ceng.solve_batch(qs)
     for q, r in zip(qs, res):
-        _check(rmat, rmat_inst, q, r, gpu_engine.solve(q.gids))
+        _check(rmat, rmat_inst, q, r, ceng.solve(q.gids))
 
 
-def test_cohorts_mixed_k_and_hop(gpu_engine, rmat, rmat_inst):
+def test_cohorts_mixed_k_and_hop(ceng, rmat, rmat_inst):
     """One batch: cohorts of k = 2..7 (k = 7 at up to 29 members per cohort, so 40 of
     them form two cohorts) beside diameter-bounded queries, which run on their own."""
     qs = _queries(rmat, [2, 3, 4, 6], 5, 7) + _queries(rmat, [7], 40, 8) + _queries(rmat, [3, 5], 4, 9, D=3)
@@ -45,9 +53,9 @@ def test_cohorts_mixed_k_and_hop(gpu_engine, rmat, rmat_inst):
     order = rng.permutation(len(qs))
     qs = [qs[i] for i in order]
     print("mixed batch of", len(qs), flush=True)
-    res = gpu_engine.solve_batch(qs)
+    res = ceng.solve_batch(qs)
     for q, r in zip(qs, res):
-        _check(rmat, rmat_inst, q, r, gpu_engine.solve(q.gids, D=q.D))
+        _check(rmat, rmat_inst, q, r, ceng.solve(q.gids, D=q.D))
 
 
 def test_cohort_equals_single_jobs(rmat):
@@ -55,7 +63,7 @@ def test_cohort_equals_single_jobs(rmat):
     same costs; without pruning, the same trees (the arenas then hold the exact DP values
     and the rebuild's choices are canonical)."""
     qs = _queries(rmat, [3, 4, 5], 6, 31)
-    with engine.Engine(rmat.graph, rmat.groups, device=0) as a, \
+    with engine.Engine(rmat.graph, rmat.groups, device=0, cohort_min=2) as a, \
             engine.Engine(rmat.graph, rmat.groups, device=0, cohort_max=1) as b:
         for flags in (0, engine.F_NO_PRUNE):
             qf = [engine.Query(q.gids, q.D, flags) for q in qs]
@@ -68,11 +76,11 @@ def test_cohort_equals_single_jobs(rmat):
 
 
 @pytest.mark.parametrize("k", [2, 4, 6])
-def test_cohort_stats_count_unpruned(gpu_engine, rmat, k):
+def test_cohort_stats_count_unpruned(ceng, rmat, k):
     """Unpruned, on the connected R-MAT graph, every state of every member is finite:
     n (2^k - 1) per member (levels 1..k-1 come back from the cohort block)."""
     qs = [engine.Query(q.gids, -1, engine.F_STATS | engine.F_NO_PRUNE) for q in _queries(rmat, [k], 5, 50 + k)]
-    res = gpu_engine.solve_batch(qs)
+    res = ceng.solve_batch(qs)
     for q, r in zip(qs, res):
         assert r["states_finite"] == rmat.graph.n * (2 ** k - 1), (q.gids, r["states_finite"])
-        assert r["cost"] == gpu_engine.solve(q.gids)["cost"]
+        assert r["cost"] == ceng.solve(q.gids)["cost"]
