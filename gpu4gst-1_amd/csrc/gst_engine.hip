@@ -264,7 +264,7 @@ void fill_jobconst(const gst_ctx *c, const Slot &s, const Query &q, JobConst &jc
   for (int l = 1, cb = q.K; l <= q.K; cb = cb * (q.K - l) / (l + 1), l++) cmax = std::max(cmax, cb);
   jc.tv = jc.hop ? std::max(1, std::min({256, HMERGE_LDS_WORDS / (jc.H * ws), HMERGE_MASKS / jc.H,
                                          SUH * BLK / (jc.H * cmax)}))
-                 : std::min(512, MERGE_LDS_WORDS / ws);
+                 : std::min({512, MERGE_LDS_WORDS / ws, SU * BLK / cmax});
   if (c->tile_k[q.K] > 0) jc.tv = std::min(jc.tv, c->tile_k[q.K]);
   int cnt[8] = {0};
   for (int p = 0; p < jc.W; p++) {

@@ -1392,8 +1392,8 @@ __host__ __device__ constexpr int merge_row_stride(int W) { return W + 1; }
 // Stages levels 1..L-1 of the tile's vertices [v0, v0 + tv) into LDS as
 // rows[(vi * H + d) * WS + p], WS = merge_row_stride(W). Each level is one contiguous
 // segment of its level block whose word i is row i / C(k,l), subset rank i % C(k,l); a
-// segment holds tv * H * C(k,l) <= 4160 * C(k,l) / (W + 1) < 2048 = SU * BLK words
-// (C(k,l) <= 2^(k-1)), so every lane issues the loads of all levels (at most
+// segment holds tv * H * C(k,l) <= 2048 = SU * BLK words (the tile size is clamped to
+// that), so every lane issues the loads of all levels (at most
 // (MAXK-1) * SU) before the first LDS store: one memory latency per tile. lsub is the
 // job's subset table, copied to LDS.
 constexpr int SU = 8;
@@ -1456,7 +1456,10 @@ __device__ __forceinline__ void flush_merge(const JobConst *jcs, JobState *jss, 
 // Non-hop merge for level L: T(v,p) = min over splits p = p1 + p2 of T(v,p1) + T(v,p2).
 // A tile of vertices has its rows staged in LDS (coalesced contiguous read), each
 // thread evaluates (vertex, subset) items, new states are activated for growth.
-constexpr int MERGE_LDS_WORDS = 4160;  // 32 rows of 129 words (k = 7), 126 of 33 (k = 5)
+// LDS of a merge tile: both merge kernels hold 5 blocks per CU by their registers, so a
+// block may take 32 KB of the CU's 160 KB (5888 words: 45 rows of 129 words at k = 7, 178
+// of 33 at k = 5) without losing a block
+constexpr int MERGE_LDS_WORDS = 5888;
 __global__ __launch_bounds__(BLK) void k_merge(const JobConst *__restrict__ jcs, JobState *jss, const Plan *pl, GraphDev g) {
   __shared__ long long soff[MAXJOBS + 1];
   __shared__ int32_t rows[MERGE_LDS_WORDS];
@@ -1570,11 +1573,11 @@ __global__ __launch_bounds__(BLK) void k_merge(const JobConst *__restrict__ jcs,
 //               min(pm(nd,p1) + pm(b,p2), pm(b,p1) + pm(nd,p2)),  b = min(nd, D - nd),
 // the best pair of heights (d1, d2) with max(d1, d2) <= nd and d1 + d2 <= D. The minimum
 // over heights of the full states, the answer, is the same as with exact heights.
-constexpr int HMERGE_LDS_WORDS = 4160;  // e.g. 5 vertices x 6 heights x 129 words (k = 7, D = 5)
-constexpr int HMERGE_MASKS = 512;       // tv * H <= 4160 / (W + 1) for W >= 8; k = 2 tiles are clamped
+constexpr int HMERGE_LDS_WORDS = 5888;  // e.g. 7 vertices x 6 heights x 129 words (k = 7, D = 5)
+constexpr int HMERGE_MASKS = 512;       // (vertex, height) masks of a tile: tv * H is clamped to it
 // Registers that carry the next tile's rows while the block evaluates the current one:
-// a hop tile's level segment holds at most 1464 words (k = 4, D = 0, level 2), so SUH
-// loads per lane and level cover it.
+// the tile size is clamped so that a level segment holds at most SUH * BLK words (1536,
+// e.g. k = 4, D = 0, level 2: 256 vertices x 6 subsets), so SUH loads per lane cover it.
 constexpr int SUH = 6;
 
 // Loads levels 1..L-1 of the rows of vertices [v0, v0 + tv) of job c into x (word i of
