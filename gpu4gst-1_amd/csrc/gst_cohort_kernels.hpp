@@ -20,14 +20,12 @@ namespace gst {
 
 constexpr int CMAXB = 64;   // members per cohort
 constexpr int CMAXCH = 16;  // 64-label chunks per cohort row (row width <= 1024 labels)
-constexpr int SX_PULL = 24; // counter stripe slot: arcs scanned by pull rounds
 
 struct CohortConst {
   int32_t *CT;               // [n][S] states of the current level
   unsigned long long *cact;  // [2][n][NW] label masks per list (bit l = label l of the row)
   int32_t *flag;             // [2][n] vertex activated in this list (next frontier)
   int32_t *items;            // [2][icap][2] (vertex, chunk) frontier items; chunk -1 = all arcs
-  int32_t *allitems;         // [icap][2] items of every vertex (the targets of a pull round)
   unsigned long long *stx;   // [NSTRIPE][NSTX] cohort work counters
   long long icap;
   const int32_t *nchunk;     // the members' grow tuning (one k, so one tuning)
@@ -39,8 +37,6 @@ struct CohortConst {
 struct CohortState {
   int32_t par;     // list consumed this round
   int32_t cnt[2];  // items per list
-  int32_t nall;    // items of allitems
-  unsigned long long arcs[2];  // arcs of the vertices of each list
 };
 
 // Bits [l0, l0 + C) of a row's label masks (C <= 64).
@@ -87,28 +83,8 @@ __global__ __launch_bounds__(BLK) void k_ccollect(const CohortConst *__restrict_
       cnt = nch ? nch : 1;
     }
     const int pos = wave_append(&cs->cnt[nx], cnt);
-    unsigned long long deg = cnt ? (unsigned long long)(g.beg[v + 1] - g.beg[v]) : 0ull;
     if (cnt) {
       gi32 *it = as_global(items) + (size_t)pos * 2;
-      if (!nch) { it[0] = (int32_t)v; it[1] = -1; }
-      for (int i = 0; i < nch; i++) { it[2 * i] = (int32_t)v; it[2 * i + 1] = i; }
-    }
-    for (int off = 32; off > 0; off >>= 1) deg += __shfl_xor(deg, off, 64);
-    if (lane_id() == 0 && deg) atomicAdd(&cs->arcs[nx], deg);
-  }
-}
-
-// Items of every vertex (as k_ccollect would list them all), for pull rounds.
-__global__ __launch_bounds__(BLK) void k_callitems(const CohortConst *__restrict__ ccp, CohortState *cs, GraphDev g) {
-  const CohortConst &cc = *ccp;
-  const long long stride = (long long)gridDim.x * BLK;
-  for (long long b = (long long)blockIdx.x * BLK + (threadIdx.x & ~63); b < g.n; b += stride) {
-    const long long v = b + lane_id();
-    const int nch = v < g.n ? chunks_of(cc.nchunk, cc.n_heavy, (int32_t)v) : 0;
-    const int cnt = v < g.n ? (nch ? nch : 1) : 0;
-    const int pos = wave_append(&cs->nall, cnt);
-    if (cnt) {
-      gi32 *it = as_global(cc.allitems) + (size_t)pos * 2;
       if (!nch) { it[0] = (int32_t)v; it[1] = -1; }
       for (int i = 0; i < nch; i++) { it[2 * i] = (int32_t)v; it[2 * i + 1] = i; }
     }
@@ -167,7 +143,9 @@ __global__ __launch_bounds__(BLK) void k_cselect(const CohortConst *__restrict__
 // grown (in an optimal tree rooted at its weighted centroid every grow step starts from a
 // subtree of weight <= OPT/2). Targets are stored whatever their cost: a grow result
 // above OPT/2 can still be a merge input at the centroid.
-constexpr int CG_AU = 4;
+#ifndef CG_AU
+#define CG_AU 4
+#endif
 __global__ __launch_bounds__(BLK) void k_cgrow(const CohortConst *__restrict__ ccp, const CohortState *__restrict__ csp,
                                                const JobConst *jcs, const JobState *jss, GraphDev g) {
   const CohortConst &cc = *ccp;
@@ -266,114 +244,6 @@ __global__ __launch_bounds__(BLK) void k_cgrow(const CohortConst *__restrict__ c
     atomicAdd(st + ST_RELAX, n_relax);
     atomicAdd(st + SX_LEVEL + 2 * (cc.L & 7), n_edges);
     atomicAdd(st + SX_LEVEL + 2 * (cc.L & 7) + 1, n_relax);
-  }
-}
-
-// Pull round, for frontiers that cover most arcs: a wavefront per target item (vertex u,
-// chunk of arcs) of every vertex, lanes over labels; for each arc (v, w) the source's
-// masks of the current list are loaded (one lane per 64-label chunk), then v's values of
-// its active chunks, CP_AU arcs in flight at a time; the minima over the arcs are
-// compared with u's row and the improvements stored. A vertex of one item owns its row
-// and its next-list masks in this round (plain stores); the chunks of a hub write with
-// atomics. Values are the same as the push rounds' (label-correcting relaxations of the
-// same selected sources); no atomic per improvement.
-constexpr int CP_AU = 4;
-__global__ __launch_bounds__(BLK) void k_cpull(const CohortConst *__restrict__ ccp, const CohortState *__restrict__ csp,
-                                               GraphDev g) {
-  const CohortConst &cc = *ccp;
-  const int cur = csp->par, nx = cur ^ 1;
-  const long long total = csp->nall;
-  const int lane = lane_id();
-  const int S = cc.S, NW = cc.NW;
-  const long long waves = (long long)gridDim.x * (BLK / WAVE);
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  gi32 *CT = as_global(cc.CT);
-  const gu64 *actc = as_global(cc.cact + (size_t)cur * g.n * NW);
-  gu64 *actn = as_global(cc.cact + (size_t)nx * g.n * NW);
-  gi32 *flagn = as_global(cc.flag + (size_t)nx * g.n);
-  const gi32 *itm = as_global(cc.allitems);
-  const int32_t *__restrict__ adj = g.adj;
-  const int32_t *__restrict__ wts = g.w;
-  unsigned long long n_edges = 0;
-  for (long long it = (long long)blockIdx.x * (BLK / WAVE) + wave; it < total; it += waves) {
-    const int32_t u = __builtin_amdgcn_readfirstlane(itm[2 * it]);
-    const int32_t ch = __builtin_amdgcn_readfirstlane(itm[2 * it + 1]);
-    const int32_t ub_ = g.beg[u], ue = g.beg[u + 1];
-    const int32_t eb = ch < 0 ? ub_ : ub_ + ch * cc.chunk;
-    const int32_t ee = ch < 0 ? ue : min(ue, eb + cc.chunk);
-    int32_t acc[CMAXCH];
-#pragma unroll
-    for (int c = 0; c < CMAXCH; c++) acc[c] = INF;
-    bool got = false;  // some arc had an active source (uniform)
-    for (int32_t e0 = eb; e0 < ee; e0 += WAVE) {
-      const int na = min(WAVE, ee - e0);
-      const int32_t av = lane < na ? adj[e0 + lane] : 0;
-      const int32_t aw = lane < na ? wts[e0 + lane] : 0;
-      for (int j = 0; j < na; j += CP_AU) {  // uniform trip count
-        int32_t v[CP_AU], wt[CP_AU];
-        unsigned long long mw[CP_AU];
-#pragma unroll
-        for (int a = 0; a < CP_AU; a++) {
-          const bool ok = j + a < na;
-          v[a] = __builtin_amdgcn_readlane(av, min(j + a, WAVE - 1));
-          wt[a] = __builtin_amdgcn_readlane(aw, min(j + a, WAVE - 1));
-          mw[a] = (ok && lane < NW) ? actc[(size_t)v[a] * NW + lane] : 0ull;  // lane c: chunk c's mask
-        }
-        int32_t x[CP_AU][CMAXCH];
-#pragma unroll
-        for (int a = 0; a < CP_AU; a++) {
-#pragma unroll
-          for (int c = 0; c < CMAXCH; c++) {
-            unsigned long long m = 0;
-            if (c < NW) {
-              m = ((unsigned long long)(uint32_t)__builtin_amdgcn_readlane((int)(mw[a] >> 32), c) << 32) |
-                  (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)mw[a], c);
-            }
-            x[a][c] = ((m >> lane) & 1) ? CT[(size_t)v[a] * S + c * 64 + lane] : INF;
-            got = got || m != 0;
-          }
-        }
-#pragma unroll
-        for (int a = 0; a < CP_AU; a++)
-#pragma unroll
-          for (int c = 0; c < CMAXCH; c++) {
-            const int nc = x[a][c] + wt[a];  // < 2^31: INF + w fits
-            acc[c] = nc < acc[c] ? nc : acc[c];
-          }
-      }
-    }
-    n_edges += ee - eb;
-    if (!got) continue;
-    unsigned long long imc = 0;
-    bool any = false;
-#pragma unroll
-    for (int c = 0; c < CMAXCH; c++) {
-      if (c >= NW) continue;
-      const int nv = acc[c] < INF ? acc[c] : INF;
-      bool imp = false;
-      if (__any(nv < INF)) {
-        gi32 *p = &CT[(size_t)u * S + c * 64 + lane];
-        imp = nv < *p;
-        if (imp) {
-          if (ch < 0) *p = nv;
-          else __hip_atomic_fetch_min(p, nv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-      }
-      const unsigned long long im = __ballot(imp);
-      if (lane == c) imc = im;
-      any = any || im != 0;
-    }
-    if (any) {
-      if (imc) {
-        if (ch < 0) actn[(size_t)u * NW + lane] = imc;
-        else __hip_atomic_fetch_or(&actn[(size_t)u * NW + lane], imc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      }
-      if (lane == 0) flagn[u] = 1;
-    }
-  }
-  if (lane == 0 && n_edges) {
-    unsigned long long *st = cc.stx + (blockIdx.x & (NSTRIPE - 1)) * NSTX;
-    atomicAdd(st + SX_PULL, n_edges);
   }
 }
 

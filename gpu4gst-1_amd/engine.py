@@ -41,7 +41,7 @@ class GstOpts(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int32), ("degree_split", ctypes.c_int32), ("chunk_edges", ctypes.c_int32),
                 ("max_batch", ctypes.c_int32), ("dp_budget_bytes", ctypes.c_int64), ("flags", ctypes.c_int32),
                 ("graph_rounds", ctypes.c_int32), ("near_far_delta", ctypes.c_int32), ("cohort_max", ctypes.c_int32),
-                ("cohort_min", ctypes.c_int32)]
+                ("cohort_min", ctypes.c_int32), ("cohort_levels", ctypes.c_int32)]
 
 
 class GstQuery(ctypes.Structure):
@@ -137,7 +137,8 @@ class Engine:
     """One resident graph + group table on one GPU."""
 
     def __init__(self, graph, groups, device=0, degree_split=0, chunk_edges=0, max_batch=0,
-                 dp_budget_bytes=0, flags=0, graph_rounds=0, near_far_delta=0, cohort_max=0, cohort_min=0):
+                 dp_budget_bytes=0, flags=0, graph_rounds=0, near_far_delta=0, cohort_max=0, cohort_min=0,
+                 cohort_levels=0):
         L = lib()
         self._beg = np.ascontiguousarray(graph.beg_pos, np.int64)
         self._adj = np.ascontiguousarray(graph.adj, np.int32)
@@ -148,7 +149,7 @@ class Engine:
                      self._w.ctypes.data)
         grp = GstGroups(len(self._goffs) - 1, self._goffs.ctypes.data, self._gverts.ctypes.data)
         opts = GstOpts(device, degree_split, chunk_edges, max_batch, dp_budget_bytes, flags, graph_rounds,
-                       near_far_delta, cohort_max, cohort_min)
+                       near_far_delta, cohort_max, cohort_min, cohort_levels)
         h = ctypes.c_void_p()
         rc = L.gst_open(ctypes.byref(csr), ctypes.byref(grp), ctypes.byref(opts), ctypes.byref(h))
         if rc != GST_OK:

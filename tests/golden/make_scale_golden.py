@@ -77,15 +77,39 @@ def run(name, threads):
     t0 = time.time()
     wl = datasets.make_workload(name)
     inst = pyoracle.Instance(wl.graph, wl.groups)
-    out = {"workload": name, "n": wl.graph.n, "m": wl.graph.m, "fingerprint": fingerprint(wl), "samples": {}}
-    print(f"{name}: generated in {time.time() - t0:.0f}s", flush=True)
-    for kind, qs in samples(wl).items():
-        rows = []
+    fp = fingerprint(wl)
+    path = os.path.join(HERE, f"scale_{name}.json")
+    # rows already solved by an earlier, interrupted run on the same graph are kept: the
+    # file is rewritten after every (k, D) group, marked "partial" until the last one
+    done = {}
+    if os.path.exists(path):
+        with open(path) as f:
+            old = json.load(f)
+        if old.get("fingerprint") == fp:
+            for kind, rows in old["samples"].items():
+                for r in rows:
+                    if "status" in r:
+                        done[(kind, tuple(r["gids"]), r["D"])] = r
+    out = {"workload": name, "n": wl.graph.n, "m": wl.graph.m, "fingerprint": fp, "samples": {}, "partial": True}
+    print(f"{name}: generated in {time.time() - t0:.0f}s; {len(done)} rows kept", flush=True)
+
+    def write():
+        with open(path, "w") as f:
+            json.dump(out, f, indent=1)
+
+    todo = samples(wl)
+    for kind, qs in todo.items():
+        rows = [{"gids": [int(v) for v in x], "D": D} for x, D in qs]
+        for r in rows:
+            r.update(done.get((kind, tuple(r["gids"]), r["D"]), {}))
+        out["samples"][kind] = rows
+    for kind, qs in todo.items():
+        rows = out["samples"][kind]
         # one oracle batch per (k, D), one query per thread; large hop arenas run fewer at once
         groups = {}
         for i, (x, D) in enumerate(qs):
-            groups.setdefault((len(x), D), []).append(i)
-        res = {}
+            if "status" not in rows[i]:
+                groups.setdefault((len(x), D), []).append(i)
         for (k, D), idx in sorted(groups.items()):
             per_query = oracle_bytes(wl.graph.n, k, D)
             if per_query > ORACLE_BYTES:
@@ -95,21 +119,14 @@ def run(name, threads):
             outs = pyoracle.batch(inst, np.stack([qs[i][0] for i in idx]), algo=0 if D < 0 else 1, D=max(D, 0),
                                   n_threads=nt)
             for i, o in zip(idx, outs):
-                res[i] = o
+                rows[i]["status"] = int(o["status"])
+                rows[i]["cost"] = int(o["tree_cost"]) if o["status"] == 0 else None
+                rows[i]["oracle_search_s"] = round(o["t_search_s"], 3)
+                rows[i]["oracle_total_s"] = round(o["t_total_s"], 3)
+            write()
             print(f"  {kind} k={k} D={D}: {len(idx)} queries on {nt} threads, {time.time() - t:.0f}s", flush=True)
-        for i, (x, D) in enumerate(qs):
-            row = {"gids": [int(v) for v in x], "D": D}
-            if i in res:
-                o = res[i]
-                row["status"] = int(o["status"])
-                row["cost"] = int(o["tree_cost"]) if o["status"] == 0 else None
-                row["oracle_search_s"] = round(o["t_search_s"], 3)
-                row["oracle_total_s"] = round(o["t_total_s"], 3)
-            rows.append(row)
-        out["samples"][kind] = rows
-    path = os.path.join(HERE, f"scale_{name}.json")
-    with open(path, "w") as f:
-        json.dump(out, f, indent=1)
+    del out["partial"]
+    write()
     print(f"wrote {path} ({time.time() - t0:.0f}s)", flush=True)
 
 

@@ -1,8 +1,10 @@
 """Cohorts: queued non-hop queries of one k grown together over a member-interleaved level
 block (the batched and served forms), then handed to the per-query jobs. Every member's
 result equals the same query solved alone and the CPU oracle; trees are valid; the
-finite-state count pins that every state of every member was written. The engine here
-forms cohorts from two queries on (the default waits for 32)."""
+finite-state count pins that every state of every member was written. The engines here
+form cohorts from two queries on (the default waits for 32); `ceng_all` keeps a cohort
+together through level k-1 (the merges of its members then write into the cohort's level
+block, and every level but the last comes back from it)."""
 import numpy as np
 import pytest
 
@@ -15,6 +17,13 @@ pytestmark = pytest.mark.gpu
 @pytest.fixture(scope="module")
 def ceng(rmat):
     eng = engine.Engine(rmat.graph, rmat.groups, device=0, cohort_min=2)
+    yield eng
+    eng.close()
+
+
+@pytest.fixture(scope="module")
+def ceng_all(rmat):
+    eng = engine.Engine(rmat.graph, rmat.groups, device=0, cohort_min=2, cohort_levels=7)
     yield eng
     eng.close()
 
@@ -37,25 +46,29 @@ def _check(rmat, inst, q, r, alone):
     assert ok, (q.gids, why)
 
 
-def test_cohort_k5(ceng, rmat, rmat_inst):
+@pytest.mark.parametrize("levels", ["1", "all"])
+def test_cohort_k5(ceng, ceng_all, rmat, rmat_inst, levels):
+    eng = ceng if levels == "1" else ceng_all
     qs = _queries(rmat, [5], 24, 2024)
-    print("cohort of", len(qs), "k=5 queries", flush=True)
-    res = ceng.solve_batch(qs)
+    print("cohort of", len(qs), "k=5 queries, levels", levels, flush=True)
+    res = eng.solve_batch(qs)
     for q, r in zip(qs, res):
-        _check(rmat, rmat_inst, q, r, ceng.solve(q.gids))
+        _check(rmat, rmat_inst, q, r, eng.solve(q.gids))
 
 
-def test_cohorts_mixed_k_and_hop(ceng, rmat, rmat_inst):
+@pytest.mark.parametrize("levels", ["1", "all"])
+def test_cohorts_mixed_k_and_hop(ceng, ceng_all, rmat, rmat_inst, levels):
     """One batch: cohorts of k = 2..7 (k = 7 at up to 29 members per cohort, so 40 of
     them form two cohorts) beside diameter-bounded queries, which run on their own."""
+    eng = ceng if levels == "1" else ceng_all
     qs = _queries(rmat, [2, 3, 4, 6], 5, 7) + _queries(rmat, [7], 40, 8) + _queries(rmat, [3, 5], 4, 9, D=3)
     rng = np.random.default_rng(10)
     order = rng.permutation(len(qs))
     qs = [qs[i] for i in order]
-    print("mixed batch of", len(qs), flush=True)
-    res = ceng.solve_batch(qs)
+    print("mixed batch of", len(qs), "levels", levels, flush=True)
+    res = eng.solve_batch(qs)
     for q, r in zip(qs, res):
-        _check(rmat, rmat_inst, q, r, ceng.solve(q.gids, D=q.D))
+        _check(rmat, rmat_inst, q, r, eng.solve(q.gids, D=q.D))
 
 
 def test_cohort_equals_single_jobs(rmat):
@@ -75,12 +88,14 @@ def test_cohort_equals_single_jobs(rmat):
                     assert np.array_equal(x["edges"], y["edges"]), q.gids
 
 
+@pytest.mark.parametrize("levels", ["1", "all"])
 @pytest.mark.parametrize("k", [2, 4, 6])
-def test_cohort_stats_count_unpruned(ceng, rmat, k):
+def test_cohort_stats_count_unpruned(ceng, ceng_all, rmat, k, levels):
     """Unpruned, on the connected R-MAT graph, every state of every member is finite:
-    n (2^k - 1) per member (levels 1..k-1 come back from the cohort block)."""
+    n (2^k - 1) per member, whichever levels come back from the cohort block."""
+    eng = ceng if levels == "1" else ceng_all
     qs = [engine.Query(q.gids, -1, engine.F_STATS | engine.F_NO_PRUNE) for q in _queries(rmat, [k], 5, 50 + k)]
-    res = ceng.solve_batch(qs)
+    res = eng.solve_batch(qs)
     for q, r in zip(qs, res):
         assert r["states_finite"] == rmat.graph.n * (2 ** k - 1), (q.gids, r["states_finite"])
-        assert r["cost"] == ceng.solve(q.gids)["cost"]
+        assert r["cost"] == eng.solve(q.gids)["cost"]
