@@ -57,6 +57,8 @@ def load_scale(name):
 
     with open(os.path.join(os.path.dirname(__file__), "golden", f"scale_{name}.json")) as f:
         fx = json.load(f)
+    if fx.get("partial"):
+        pytest.skip(f"scale_{name}.json is incomplete (its generator was interrupted; run it again to finish it)")
     wl = datasets.make_workload(name)
     assert fingerprint(wl) == fx["fingerprint"], f"{name}: generated graph differs from the fixture's"
     return wl, fx
