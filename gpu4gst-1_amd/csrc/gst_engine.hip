@@ -126,7 +126,8 @@ struct gst_ctx {
   unsigned long long *d_count = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;  // bracket the work kernels of one round
   unsigned merge_grid = 1024;               // resident blocks of k_merge on this device
-  unsigned merge_hop_grid = 1024;           // resident blocks of k_merge_hop on this device
+  unsigned merge_hop_grid = 1024;           // resident blocks of k_merge_hop<false> on this device
+  unsigned merge_hopb_grid = 1024;          // resident blocks of k_merge_hop<true>
   int max_batch = 64;
   // batched / served (gst_batch.inc)
   struct Batch;
@@ -266,6 +267,14 @@ void fill_jobconst(const gst_ctx *c, const Slot &s, const Query &q, JobConst &jc
                                          SUH * BLK / (jc.H * cmax)}))
                  : std::min({512, MERGE_LDS_WORDS / ws, SU * BLK / cmax});
   if (c->tile_k[q.K] > 0) jc.tv = std::min(jc.tv, c->tile_k[q.K]);
+  // bound-first hop tiles (levels 4..k-1, pruned): whole subtiles of tv vertices whose
+  // singletons (H * k words per vertex) fit the registers that carry a tile's level 1, the
+  // merge LDS and the activation masks; at most 16 subtiles
+  jc.tvb = jc.tv;
+  if (jc.hop) {
+    const int cap = std::min({SUH * BLK, HMERGE_LDS_WORDS} ) / (jc.H * q.K);
+    jc.tvb = jc.tv * std::max(1, std::min({16, cap / jc.tv, HMERGE_MASKS / jc.H / jc.tv}));
+  }
   int cnt[8] = {0};
   for (int p = 0; p < jc.W; p++) {
     const int L = __builtin_popcount(p);
@@ -592,8 +601,10 @@ int gst_open(const gst_csr *csr, const gst_groups *groups, const gst_opts *opts,
     // resident blocks leaves no partial second wave of blocks behind the first
     int ncu = 0, nb = 0;
     HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->dev));
-    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_merge_hop, BLK, 0));
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_merge_hop<false>, BLK, 0));
     c->merge_hop_grid = (unsigned)std::max(1, ncu * nb);
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_merge_hop<true>, BLK, 0));
+    c->merge_hopb_grid = (unsigned)std::max(1, ncu * nb);
     HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_merge, BLK, 0));
     c->merge_grid = (unsigned)std::max(1, ncu * nb);
   }
@@ -840,9 +851,12 @@ static int solve_on_slot(gst_ctx *c, Slot &s, const Query &q, int32_t *ebuf, int
         k_tables<<<1, BLK, 0, c->st>>>(c->d_jc, c->d_js, 1, g);
         break;
       case PH_MERGE: {
-        const long long tiles = ((long long)c->n + jc.tv - 1) / jc.tv;
+        const bool bf = hop_bound_first(jc, h.level);
+        const int tile = bf ? jc.tvb : jc.tv;
+        const long long tiles = ((long long)c->n + tile - 1) / tile;
         HIPCHK(hipEventRecord(c->ev0, c->st));
-        if (q.D >= 0) k_merge_hop<<<grid_for(tiles, 1, c->merge_hop_grid), BLK, 0, c->st>>>(c->d_jc, c->d_js, pl, g);
+        if (bf) k_merge_hop<true><<<grid_for(tiles, 1, c->merge_hopb_grid), BLK, 0, c->st>>>(c->d_jc, c->d_js, pl, g);
+        else if (q.D >= 0) k_merge_hop<false><<<grid_for(tiles, 1, c->merge_hop_grid), BLK, 0, c->st>>>(c->d_jc, c->d_js, pl, g);
         else k_merge<<<grid_for(tiles, 1, c->merge_grid), BLK, 0, c->st>>>(c->d_jc, c->d_js, pl, g);
         if (h.level < q.K && (q.D != 0)) {
           const long long nblk = ((long long)c->n + 63) / 64 * (q.D > 0 ? q.D : 1);

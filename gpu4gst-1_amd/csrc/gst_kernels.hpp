@@ -91,6 +91,7 @@ struct JobConst {
   int32_t K, D, H, W, R, hop, flags, nl;
   int32_t seed_total;         // members of the query groups (seed work items)
   int32_t tv;                 // vertices per merge tile
+  int32_t tvb;                // vertices per bound-first hop merge tile (a multiple of tv)
   int32_t delta;              // near-far step of the grow frontier (0 = off)
   int32_t passive;            // cohort member: the grow phases run in the cohort kernels
   // grow tuning of this job (chosen per k): vertices of degree >= split are grown as
@@ -135,8 +136,9 @@ constexpr int FAR_LIST = 2;
 constexpr int FAR_ACT = 2;
 
 // Work items of one round, flattened over jobs: off[kind][j] = first item of job j.
+// WK_MERGEB: the bound-first tiles of hop merges (k_merge_hop<true>); WK_MERGE: the others.
 enum WorkKind : int32_t { WK_SEED = 0, WK_THR = 1, WK_WAVE = 2, WK_CLEAR = 3, WK_MERGE = 4, WK_FINAL = 5, WK_FAR = 6,
-                          WK_COLLECT = 7, NWK = 8 };
+                          WK_COLLECT = 7, WK_MERGEB = 8, NWK = 9 };
 struct Plan {
   int32_t njobs, pad;
   long long off[NWK][MAXJOBS + 1];
@@ -376,6 +378,12 @@ hop_entry: {
   }
 }
 
+// Hop merge tiles of levels 4..k-1 (8 or more splits per subset) take the bound-first form
+// (k_merge_hop<true>, work kind WK_MERGEB) when pruning is on, over tvb vertices per tile.
+__host__ __device__ __forceinline__ bool hop_bound_first(const JobConst &c, int L) {
+  return c.hop && !(c.flags & JF_NO_PRUNE) && L >= 4 && L < c.K;
+}
+
 // One block (BLK >= njobs threads): advance every job, then lay out this round's work
 // items per kind as exclusive prefix sums over jobs.
 __global__ __launch_bounds__(BLK) void k_step(const JobConst *__restrict__ jcs, JobState *jss, Plan *pl,
@@ -387,7 +395,7 @@ __global__ __launch_bounds__(BLK) void k_step(const JobConst *__restrict__ jcs, 
     JobState &s = jss[j];
     step_job(c, s);
     const int ph = s.phase;
-    long long k[NWK] = {0, 0, 0, 0, 0, 0, 0, 0};
+    long long k[NWK] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     if (ph == PH_SEED) k[WK_SEED] = c.seed_total;
     if (ph == PH_GROW) {
       const int li = c.hop ? s.depth : s.parity;
@@ -396,7 +404,10 @@ __global__ __launch_bounds__(BLK) void k_step(const JobConst *__restrict__ jcs, 
       k[WK_CLEAR] = k[WK_THR] + k[WK_WAVE];
     }
     if (ph == PH_HOPCLEAR) k[WK_CLEAR] = s.cnt[c.D][0] + s.cnt[c.D][1];
-    if (ph == PH_MERGE) k[WK_MERGE] = ((long long)n + c.tv - 1) / c.tv;
+    if (ph == PH_MERGE) {
+      if (hop_bound_first(c, s.level)) k[WK_MERGEB] = ((long long)n + c.tvb - 1) / c.tvb;
+      else k[WK_MERGE] = ((long long)n + c.tv - 1) / c.tv;
+    }
     // blocks of 64 vertices per list the merge activates into (heights 0..D-1, or one list)
     if (ph == PH_MERGE && s.level < c.K) k[WK_COLLECT] = (((long long)n + 63) / 64) * (c.hop ? c.D : 1);
     if (ph == PH_FINAL) k[WK_FINAL] = ((long long)n * c.H + FINAL_CHUNK - 1) / FINAL_CHUNK;
@@ -1580,16 +1591,16 @@ constexpr int HMERGE_MASKS = 512;       // (vertex, height) masks of a tile: tv 
 // e.g. k = 4, D = 0, level 2: 256 vertices x 6 subsets), so SUH loads per lane cover it.
 constexpr int SUH = 6;
 
-// Loads levels 1..L-1 of the rows of vertices [v0, v0 + tv) of job c into x (word i of
+// Loads levels lo..hi-1 of the rows of vertices [v0, v0 + tv) of job c into x (word i of
 // level l's contiguous segment in x[l - 1][i / BLK] of thread i % BLK), without waiting.
-__device__ __forceinline__ void load_hop_tile(const JobConst &c, int L, long long v0, int tv,
+__device__ __forceinline__ void load_hop_tile(const JobConst &c, int lo, int hi, long long v0, int tv,
                                               int32_t (&x)[MAXK - 1][SUH]) {
   const int t = (int)threadIdx.x;
   // the subset counts as one 8-byte scalar load
   const unsigned long long nsw = *(const unsigned long long *)c.nsub;
 #pragma unroll
   for (int l = 1; l < MAXK; l++) {
-    if (l < L) {
+    if (l >= lo && l < hi) {
       const int ns = (int)((nsw >> (8 * l)) & 0xff), n = tv * c.H * ns;
       const gi32 *src = as_global(c.T + c.off[l] + v0 * c.H * ns);
       // every lane loads (past the segment end: its last word, never stored): a
@@ -1602,9 +1613,10 @@ __device__ __forceinline__ void load_hop_tile(const JobConst &c, int L, long lon
 }
 
 // Issues the loads of merge tile gt into x if it belongs to a hop job; returns gt, or -1.
-// The job index is made uniform and the job table is read-only in the kernel, so the
-// job's fields come through scalar loads and never wait behind the loads in flight.
-// lvl: each job's level, in LDS.
+// Bound-first tiles (BF) fetch level 1 only. The job index is made uniform and the job
+// table is read-only in the kernel, so the job's fields come through scalar loads and
+// never wait behind the loads in flight. lvl: each job's level, in LDS.
+template <bool BF>
 __device__ __forceinline__ long long issue_hop_tile(const JobConst *__restrict__ jcs, const int *lvl,
                                                     const long long *soff, int nj, long long gt, long long total,
                                                     int n, int32_t (&x)[MAXK - 1][SUH]) {
@@ -1612,29 +1624,37 @@ __device__ __forceinline__ long long issue_hop_tile(const JobConst *__restrict__
   const int jb = __builtin_amdgcn_readfirstlane(find_job(soff, nj, gt));
   const JobConst &c = jcs[jb];
   if (!c.hop) return -1;
-  const long long v0 = (gt - soff[jb]) * c.tv;
-  load_hop_tile(c, lvl[jb], v0, (int)min((long long)c.tv, (long long)n - v0), x);
+  const int tvx = BF ? c.tvb : c.tv;
+  const long long v0 = (gt - soff[jb]) * tvx;
+  load_hop_tile(c, 1, BF ? 2 : lvl[jb], v0, (int)min((long long)tvx, (long long)n - v0), x);
   return gt;
 }
 
-__global__ __launch_bounds__(BLK) void k_merge_hop(const JobConst *__restrict__ jcs, JobState *jss, const Plan *pl,
-                                                   GraphDev g) {
+// BF = false: the tiles of work kind WK_MERGE (levels 2, 3 and k, and every level when
+// pruning is off); BF = true: the bound-first tiles (WK_MERGEB), compiled for 4 waves per
+// SIMD (its own register needs are larger, and it must not lower the occupancy of the
+// other form, whose tiles are latency-bound).
+template <bool BF>
+__global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(BF ? 4 : 1))) void k_merge_hop(
+    const JobConst *__restrict__ jcs, JobState *jss, const Plan *pl, GraphDev g) {
   __shared__ long long soff[MAXJOBS + 1];
   __shared__ int lvl[MAXJOBS];
   __shared__ int32_t pm[HMERGE_LDS_WORDS];
   __shared__ unsigned long long amask[HMERGE_MASKS];
   __shared__ int ubs, ub0s;
   __shared__ uint32_t lsubw[128];
+  __shared__ int32_t hq[BF ? BLK : 1];  // items queued for their split loops, WAVE entries per wavefront
+  __shared__ unsigned live;             // subtiles of a bound-first tile with an item that passes
   uint8_t (*lsub)[64] = (uint8_t (*)[64])lsubw;
   for (int j = threadIdx.x; j < pl->njobs; j += BLK) lvl[j] = jss[j].level;
-  const int nj = load_plan(pl, WK_MERGE, soff);  // (its barrier also publishes lvl)
+  const int nj = load_plan(pl, BF ? WK_MERGEB : WK_MERGE, soff);  // (its barrier also publishes lvl)
   const long long total = soff[nj];
   int cj = -1, minc_acc = INF;
   unsigned long long merged_acc = 0;
   // A tile's rows travel in registers while the block evaluates the previous tile: the
   // loads of tile gt + gridDim.x are issued once tile gt is in LDS.
   int32_t x[MAXK - 1][SUH];
-  long long issued = issue_hop_tile(jcs, lvl, soff, nj, blockIdx.x, total, g.n, x);
+  long long issued = issue_hop_tile<BF>(jcs, lvl, soff, nj, blockIdx.x, total, g.n, x);
   for (long long gt = blockIdx.x; gt < total; gt += gridDim.x) {
   const int jb = __builtin_amdgcn_readfirstlane(find_job(soff, nj, gt));
   if (jb != cj) { flush_merge(jcs, jss, cj, merged_acc, minc_acc); cj = jb; }
@@ -1643,69 +1663,69 @@ __global__ __launch_bounds__(BLK) void k_merge_hop(const JobConst *__restrict__ 
   if (!c.hop) continue;
   const int W = c.W, H = c.H, D = c.D, K = c.K, L = lvl[jb];
   const int WS = merge_row_stride(W), RW = H * WS;
-  const int TV = c.tv;
   const int nL = c.nsub[L];
   int merged = 0;
   {
-    const long long v0 = (gt - soff[jb]) * TV;
-    const int tv = (int)min((long long)TV, (long long)g.n - v0);
-    if (issued != gt) issued = issue_hop_tile(jcs, lvl, soff, nj, gt, total, g.n, x);
-    for (int i = threadIdx.x; i < tv * H; i += BLK) amask[i] = 0;
+    if (issued != gt) issued = issue_hop_tile<BF>(jcs, lvl, soff, nj, gt, total, g.n, x);
+    const int TVX = BF ? c.tvb : c.tv;
+    const long long v0 = (gt - soff[jb]) * TVX;
+    const int tvx = (int)min((long long)TVX, (long long)g.n - v0);  // vertices of the tile
+    for (int i = threadIdx.x; i < tvx * H; i += BLK) amask[i] = 0;
     // UB known when the tile starts: UB candidates of the tile are compared with it, and
     // the growth filter of the hop select pass uses it. Read by one thread and shared
     // through LDS, so that no vector load is pending while the next tile's rows are.
-    if (threadIdx.x == 0) { ubs = INF; ub0s = load_ub(s); }
+    if (threadIdx.x == 0) { ubs = INF; ub0s = load_ub(s); live = 0; }
     load_lsub(c, lsub);
     __syncthreads();
     const int ub0 = ub0s;
-    // rows -> pm[(vi * H + d) * WS + q]; every subset read below has fewer than L members,
+    const bool prune = !(c.flags & JF_NO_PRUNE);
+    int ubl = INF;
+    // The (sub)tile being merged: vertices [v0 + vb, v0 + vb + tv); its rows of levels
+    // 1..L-1 are staged in pm[(vi * H + d) * WS + q].
+    int vb = 0, tv = tvx;
+    // rows of levels 1..L-1 in x -> pm; every subset read below has fewer than L members,
     // so this overwrites all of it. Height-D singletons are never grown (so the select
     // pass never sees them): the staged values are checked for saturation here.
-    bool satl = false;
+    auto stage = [&]() {
+      bool satl = false;
 #pragma unroll
-    for (int l = 1; l < MAXK; l++) {
-      if (l < L) {
-        const int ns = c.nsub[l], n = tv * H * ns;
-        const UDiv dn(ns);
+      for (int l = 1; l < MAXK; l++) {
+        if (l < L) {
+          const int ns = c.nsub[l], n = tv * H * ns;
+          const UDiv dn(ns);
 #pragma unroll
-        for (int u = 0; u < SUH; u++) {
-          const int i = u * BLK + (int)threadIdx.x;
-          if (i < n) {
-            const int r = (int)dn.div((uint32_t)i);
-            const int32_t xv = x[l - 1][u];
-            satl |= xv >= SAT_AT && xv < INF;
-            pm[r * WS + lsub[l][i - r * ns]] = xv;
+          for (int u = 0; u < SUH; u++) {
+            const int i = u * BLK + (int)threadIdx.x;
+            if (i < n) {
+              const int r = (int)dn.div((uint32_t)i);
+              const int32_t xv = x[l - 1][u];
+              satl |= xv >= SAT_AT && xv < INF;
+              pm[r * WS + lsub[l][i - r * ns]] = xv;
+            }
           }
         }
       }
-    }
-    if (satl) flag_saturation(s);
-    __syncthreads();
-    for (int i = threadIdx.x; i < tv * W; i += BLK) {  // prefix minima over height, in place
-      const int vi = i >> c.K, q = i & (W - 1);
-      if (q == 0 || popc(q) >= L) continue;
-      int m = INF;
-      for (int d = 0; d < H; d++) {
-        int32_t *y = &pm[vi * RW + d * WS + q];
-        m = *y < m ? *y : m;
-        *y = m;
+      if (satl) flag_saturation(s);
+    };
+    auto prefix = [&]() {  // prefix minima over height, in place
+      for (int i = threadIdx.x; i < tv * W; i += BLK) {
+        const int vi = i >> c.K, q = i & (W - 1);
+        if (q == 0 || popc(q) >= L) continue;
+        int m = INF;
+        for (int d = 0; d < H; d++) {
+          int32_t *y = &pm[vi * RW + d * WS + q];
+          m = *y < m ? *y : m;
+          *y = m;
+        }
       }
-    }
-    issued = issue_hop_tile(jcs, lvl, soff, nj, gt + gridDim.x, total, g.n, x);
-    __syncthreads();
-    const bool prune = !(c.flags & JF_NO_PRUNE);
-    int ubl = INF;
-    if (L == 2) {  // stars: arms of at most floor(D/2) hops, prefix minima of the singletons
-      for (int vi = threadIdx.x; vi < tv; vi += BLK) {
-        long long star = 0;
-        for (int i = 0; i < K; i++) star += pm[(vi * H + D / 2) * WS + (1 << i)];
-        if (star < ubl) ubl = (int)star;
-      }
-    }
-    // item order (vertex, height, subset) makes the level-block write below contiguous
+    };
+    // item order (vertex, height, subset) makes the level-block writes contiguous
     const UDiv dnl(nL), dh(H);
     const int nsplit = 1 << (L - 1);  // splits per subset of the level, plus one
-    for (int it = threadIdx.x; it < tv * nL * H; it += BLK) {
+    gi32 *const Tl0 = as_global(c.T) + (size_t)c.off[L] + (size_t)v0 * H * nL;
+    gi32 *Tl = Tl0;
+    int nitems = tv * nL * H;
+    auto merge_item = [&](int it) {
       const int row = (int)dnl.div((uint32_t)it);   // vi * H + nd
       const int j = it - row * nL;
       const int vi = (int)dh.div((uint32_t)row), nd = row - vi * H;
@@ -1744,15 +1764,140 @@ __global__ __launch_bounds__(BLK) void k_merge_hop(const JobConst *__restrict__ 
       }
       // every state of the level is written here, INF included: levels >= 2 are not
       // filled at query start
-      as_global(c.T)[(size_t)c.off[L] + (size_t)v0 * H * nL + it] = keep ? m : INF;
-      if (!keep) continue;
+      Tl[it] = keep ? m : INF;
+      if (!keep) return;
       merged++;
-      if (L < K && nd < D) atomicOr(&amask[vi * H + nd], 1ull << j);
+      if (L < K && nd < D) atomicOr(&amask[(vb + vi) * H + nd], 1ull << j);
+    };
+    if constexpr (!BF) {
+      stage();
+      __syncthreads();
+      prefix();
+      issued = issue_hop_tile<BF>(jcs, lvl, soff, nj, gt + gridDim.x, total, g.n, x);
+      __syncthreads();
+      if (L == 2) {  // stars: arms of at most floor(D/2) hops, prefix minima of the singletons
+        for (int vi = threadIdx.x; vi < tv; vi += BLK) {
+          long long star = 0;
+          for (int i = 0; i < K; i++) star += pm[(vi * H + D / 2) * WS + (1 << i)];
+          if (star < ubl) ubl = (int)star;
+        }
+      }
+      for (int it = threadIdx.x; it < nitems; it += BLK) merge_item(it);
+    } else {
+      // Bound-first tile (pruning on, 4 <= L < k) of up to tvb vertices. The bound of an
+      // item from the singletons alone: the tree of a state (v,p,nd) holds, for each group
+      // i of p, a walk of at most nd hops from v to i, and the rest of an optimal
+      // derivation through it holds a walk of at most D - nd hops to each group outside p;
+      // with level 1 unpruned, the prefix minima of the singletons are exactly those
+      // hop-bounded distances. So
+      //   max_{i in p} T1(v,i,<=nd) + max_{i not in p} T1(v,i,<=D-nd) > UB
+      // implies m + lb > UB (the state is not stored) and puts every UB candidate of the
+      // item above UB (its complement part is >= the second maximum): such an item is
+      // written INF without its splits. sg(vi, d, i) is the prefix minimum T1(v, i, <= d).
+      auto passes = [&](int it, auto sg) {
+        const int row = (int)dnl.div((uint32_t)it);
+        const int j = it - row * nL;
+        const int vi = (int)dh.div((uint32_t)row), nd = row - vi * H;
+        const int p = lsub[L][j];
+        int in = 0, out = 0;
+#pragma unroll 1
+        for (int i = 0; i < K; i++) {
+          if ((p >> i) & 1) in = max(in, sg(vi, nd, i));
+          else out = max(out, sg(vi, D - nd, i));
+        }
+        return (long long)in + out <= ub0;
+      };
+      // x holds the tile's singletons only: staged compactly as sgc[(vi * H + d) * K + i]
+      // and turned into prefix minima over height
+      {
+        const int nw = tvx * H * K;
+        bool satl = false;
+#pragma unroll
+        for (int u = 0; u < SUH; u++) {
+          const int i = u * BLK + (int)threadIdx.x;
+          if (i < nw) {
+            const int r = i / K, jj = i - r * K;
+            const int32_t xv = x[0][u];
+            satl |= xv >= SAT_AT && xv < INF;
+            pm[r * K + __builtin_ctz(lsub[1][jj])] = xv;
+          }
+        }
+        if (satl) flag_saturation(s);
+      }
+      __syncthreads();
+      for (int i = threadIdx.x; i < tvx * K; i += BLK) {
+        const int vi = i / K, gi = i - vi * K;
+        int m = INF;
+        for (int d = 0; d < H; d++) {
+          int32_t *y = &pm[(vi * H + d) * K + gi];
+          m = *y < m ? *y : m;
+          *y = m;
+        }
+      }
+      __syncthreads();
+      // the items that fail are written INF; each subtile of tv vertices with an item that
+      // passes (5-25% of them at levels 4 and 5, k = 7, on the power-law workloads) is
+      // then staged whole and its passing items merged. While the UB is still infinite
+      // every subtile is: then no answer may exist, and every staged row is checked for
+      // saturation as in the other form.
+      const int nsub = (tvx + c.tv - 1) / c.tv;
+      {
+        auto sgc = [&](int vi, int d, int i) { return pm[(vi * H + d) * K + i]; };
+        unsigned lm = 0;
+        for (int it = threadIdx.x; it < tvx * nL * H; it += BLK) {
+          if (passes(it, sgc)) lm |= 1u << ((int)dh.div((uint32_t)dnl.div((uint32_t)it)) / c.tv);
+          else Tl0[it] = INF;
+        }
+        if (lm) atomicOr(&live, lm);
+      }
+      __syncthreads();
+      const unsigned lv = ub0 == INF ? (1u << nsub) - 1 : live;
+      if (!lv) issued = issue_hop_tile<BF>(jcs, lvl, soff, nj, gt + gridDim.x, total, g.n, x);
+      auto sgr = [&](int vi, int d, int i) { return pm[(vi * H + d) * WS + (1 << i)]; };
+      for (int st = 0; st < nsub; st++) {
+        if (!((lv >> st) & 1)) continue;
+        vb = st * c.tv;
+        tv = min(c.tv, tvx - vb);
+        Tl = Tl0 + (size_t)vb * H * nL;
+        nitems = tv * nL * H;
+        load_hop_tile(c, 1, L, v0 + vb, tv, x);
+        __syncthreads();  // the bound pass or the previous subtile is done with pm
+        stage();
+        __syncthreads();
+        prefix();
+        if (lv >> (st + 1) == 0)  // the last subtile: x is free for the next tile
+          issued = issue_hop_tile<BF>(jcs, lvl, soff, nj, gt + gridDim.x, total, g.n, x);
+        __syncthreads();
+        // passing items queued per wavefront in LDS and evaluated 64 at a time, one per
+        // lane, so that their uniform split loops run on full wavefronts
+        int32_t *wq = hq + (threadIdx.x & ~(WAVE - 1));
+        const int lane = lane_id();
+        int pend = 0;  // queued items of this wavefront (uniform)
+        for (int it0 = 0; it0 < nitems; it0 += BLK) {  // uniform trip count
+          const int it = it0 + (int)threadIdx.x;
+          const bool pass = it < nitems && passes(it, sgr);
+          const unsigned long long pm_ = __ballot(pass);
+          const int np = __popcll(pm_);
+          if (pend + np > WAVE) {
+            __builtin_amdgcn_wave_barrier();
+            if (lane < pend) merge_item(wq[lane]);
+            __builtin_amdgcn_wave_barrier();
+            pend = 0;
+          }
+          if (pass)
+            wq[pend + __builtin_amdgcn_mbcnt_hi((unsigned)(pm_ >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)pm_, 0))] = it;
+          pend += np;
+        }
+        __builtin_amdgcn_wave_barrier();
+        if (lane < pend) merge_item(wq[lane]);
+        __builtin_amdgcn_wave_barrier();
+      }
+      tv = tvx;
     }
     if (ubl < ub0) atomicMin(&ubs, ubl);
     __syncthreads();
     if (threadIdx.x == 0 && ubs < ub0) atomicMin(&s->ub, ubs);
-    for (int i = threadIdx.x; i < tv * H; i += BLK) {  // k_collect builds the worklists
+    for (int i = threadIdx.x; i < tvx * H; i += BLK) {  // k_collect builds the worklists
       const unsigned long long mk = amask[i];
       if (mk) {
         const int vi = (int)dh.div((uint32_t)i), nd = i - vi * H;
