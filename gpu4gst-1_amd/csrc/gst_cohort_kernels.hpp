@@ -133,9 +133,10 @@ __global__ __launch_bounds__(BLK) void k_cselect(const CohortConst *__restrict__
 }
 
 // Grow: a wavefront per item (vertex, chunk of arcs); lanes over the row's labels, 64 per
-// chunk. v's values of its active labels stay in registers; for each arc (u, w), the
-// target row's active chunks are read with coalesced loads, CG_AU arcs in flight at a
-// time; an improvement is an atomicMin (fire-and-forget), the improved bits are OR-ed
+// chunk, NWT chunks at most (the row's chunk count rounded up to an instantiated one, so
+// that the per-arc arrays stay in registers). v's values of its active labels stay in
+// registers; for each arc (u, w), the target row's active chunks are read with coalesced
+// loads, AU arcs in flight at a time; an improvement is an atomicMin (fire-and-forget), the improved bits are OR-ed
 // into u's masks of the next list by one lane per chunk, and u is flagged for the next
 // frontier (k_ccollect builds it).
 // The UB/2 rule is applied here again, with each member's UB as it stands when the item
@@ -143,9 +144,7 @@ __global__ __launch_bounds__(BLK) void k_cselect(const CohortConst *__restrict__
 // grown (in an optimal tree rooted at its weighted centroid every grow step starts from a
 // subtree of weight <= OPT/2). Targets are stored whatever their cost: a grow result
 // above OPT/2 can still be a merge input at the centroid.
-#ifndef CG_AU
-#define CG_AU 4
-#endif
+template <int NWT, int AU>
 __global__ __launch_bounds__(BLK) void k_cgrow(const CohortConst *__restrict__ ccp, const CohortState *__restrict__ csp,
                                                const JobConst *jcs, const JobState *jss, GraphDev g) {
   const CohortConst &cc = *ccp;
@@ -166,11 +165,11 @@ __global__ __launch_bounds__(BLK) void k_cgrow(const CohortConst *__restrict__ c
   for (long long it = (long long)blockIdx.x * (BLK / WAVE) + wave; it < total; it += waves) {
     const int32_t v = __builtin_amdgcn_readfirstlane(itm[2 * it]);
     const int32_t ch = __builtin_amdgcn_readfirstlane(itm[2 * it + 1]);
-    unsigned long long m[CMAXCH];
-    int32_t sv[CMAXCH], cap[CMAXCH];  // cap: the member's floor(UB/2), the largest value grown
+    unsigned long long m[NWT];
+    int32_t sv[NWT], cap[NWT];  // cap: the member's floor(UB/2), the largest value grown
     int nb = 0;
 #pragma unroll
-    for (int c = 0; c < CMAXCH; c++) {
+    for (int c = 0; c < NWT; c++) {
       unsigned long long x = c < NW ? actc[(size_t)v * NW + c] : 0ull;
       x = ((unsigned long long)(uint32_t)__builtin_amdgcn_readfirstlane((int)(x >> 32)) << 32) |
           (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)x);
@@ -196,28 +195,28 @@ __global__ __launch_bounds__(BLK) void k_cgrow(const CohortConst *__restrict__ c
       const int na = min(WAVE, ee - e0);
       const int32_t au = lane < na ? adj[e0 + lane] : 0;
       const int32_t aw = lane < na ? wts[e0 + lane] : 0;
-      for (int j = 0; j < na; j += CG_AU) {  // uniform trip count
-        int32_t u[CG_AU], wt[CG_AU];
-        bool ok[CG_AU];
+      for (int j = 0; j < na; j += AU) {  // uniform trip count
+        int32_t u[AU], wt[AU];
+        bool ok[AU];
 #pragma unroll
-        for (int a = 0; a < CG_AU; a++) {
+        for (int a = 0; a < AU; a++) {
           ok[a] = j + a < na;
           u[a] = __builtin_amdgcn_readlane(au, min(j + a, WAVE - 1));
           wt[a] = __builtin_amdgcn_readlane(aw, min(j + a, WAVE - 1));
         }
-        int32_t t[CG_AU][CMAXCH];
+        int32_t t[AU][NWT];
 #pragma unroll
-        for (int a = 0; a < CG_AU; a++)
+        for (int a = 0; a < AU; a++)
 #pragma unroll
-          for (int c = 0; c < CMAXCH; c++)
+          for (int c = 0; c < NWT; c++)
             t[a][c] = (ok[a] && ((m[c] >> lane) & 1)) ? CT[(size_t)u[a] * S + c * 64 + lane] : 0;
 #pragma unroll
-        for (int a = 0; a < CG_AU; a++) {
+        for (int a = 0; a < AU; a++) {
           if (!ok[a]) continue;
           unsigned long long imc = 0;  // lane c: improved bits of chunk c
           bool any = false;
 #pragma unroll
-          for (int c = 0; c < CMAXCH; c++) {
+          for (int c = 0; c < NWT; c++) {
             if (!m[c]) continue;  // uniform
             int nc = sv[c] + wt[a];
             nc = nc < INF ? nc : INF;
