@@ -142,8 +142,10 @@ __global__ __launch_bounds__(BLK) void k_cselect(const CohortConst *__restrict__
 // The UB/2 rule is applied here again, with each member's UB as it stands when the item
 // starts (it only decreases after the select pass): a source above floor(UB/2) is not
 // grown (in an optimal tree rooted at its weighted centroid every grow step starts from a
-// subtree of weight <= OPT/2). Targets are stored whatever their cost: a grow result
-// above OPT/2 can still be a merge input at the centroid.
+// subtree of weight <= OPT/2). Targets up to the member's UB are stored (a grow result
+// above OPT/2 can still be a merge input at the centroid); a relaxation that would lift a
+// label above its member's UB is not made, and since rows are sorted by weight the arc
+// scan of an item ends at the first arc too heavy for every grown label.
 template <int NWT, int AU>
 __global__ __launch_bounds__(BLK) void k_cgrow(const CohortConst *__restrict__ ccp, const CohortState *__restrict__ csp,
                                                const JobConst *jcs, const JobState *jss, GraphDev g) {
@@ -167,7 +169,8 @@ __global__ __launch_bounds__(BLK) void k_cgrow(const CohortConst *__restrict__ c
     const int32_t ch = __builtin_amdgcn_readfirstlane(itm[2 * it + 1]);
     unsigned long long m[NWT];
     int32_t sv[NWT], cap[NWT];  // cap: the member's floor(UB/2), the largest value grown
-    int nb = 0;
+    int32_t lim[NWT];           // the heaviest arc a grown label relaxes: UB - its value
+    int nb = 0, lmax = -1;
 #pragma unroll
     for (int c = 0; c < NWT; c++) {
       unsigned long long x = c < NW ? actc[(size_t)v * NW + c] : 0ull;
@@ -176,40 +179,55 @@ __global__ __launch_bounds__(BLK) void k_cgrow(const CohortConst *__restrict__ c
       const bool bit = (x >> lane) & 1;
       sv[c] = bit ? CT[(size_t)v * S + c * 64 + lane] : INF;
       cap[c] = INF;
+      int ubq = INF;
       if (bit) {
         const int q = (c * 64 + lane) / cc.C;
-        if (!(jcs[cc.jobs[q]].flags & JF_NO_PRUNE)) cap[c] = load_ub(&jss[cc.jobs[q]]) >> 1;
+        if (!(jcs[cc.jobs[q]].flags & JF_NO_PRUNE)) {
+          ubq = load_ub(&jss[cc.jobs[q]]);
+          cap[c] = ubq >> 1;
+        }
       }
       const bool grow = bit && sv[c] <= cap[c];
+      lim[c] = grow ? ubq - sv[c] : -1;
+      lmax = lim[c] > lmax ? lim[c] : lmax;
       x = __ballot(grow);
       m[c] = x;
       nb += __popcll(x);
     }
     if (!nb) continue;
+    // rows are sorted by weight: the scan ends at the first arc heavier than every lim
+    for (int off = 32; off > 0; off >>= 1) {
+      const int o = __shfl_xor(lmax, off, 64);
+      lmax = o > lmax ? o : lmax;
+    }
+    lmax = __builtin_amdgcn_readfirstlane(lmax);
     const int32_t vb = g.beg[v], ve = g.beg[v + 1];
     const int32_t eb = ch < 0 ? vb : vb + ch * cc.chunk;
     const int32_t ee = ch < 0 ? ve : min(ve, eb + cc.chunk);
-    n_edges += ee - eb;
-    n_relax += (unsigned long long)(ee - eb) * nb;
     for (int32_t e0 = eb; e0 < ee; e0 += WAVE) {
       const int na = min(WAVE, ee - e0);
       const int32_t au = lane < na ? adj[e0 + lane] : 0;
       const int32_t aw = lane < na ? wts[e0 + lane] : 0;
+      if (__builtin_amdgcn_readfirstlane(aw) > lmax) break;  // lane 0 holds arc e0
+      n_edges += na;
       for (int j = 0; j < na; j += AU) {  // uniform trip count
         int32_t u[AU], wt[AU];
         bool ok[AU];
 #pragma unroll
         for (int a = 0; a < AU; a++) {
-          ok[a] = j + a < na;
           u[a] = __builtin_amdgcn_readlane(au, min(j + a, WAVE - 1));
           wt[a] = __builtin_amdgcn_readlane(aw, min(j + a, WAVE - 1));
+          ok[a] = j + a < na && wt[a] <= lmax;
         }
-        int32_t t[AU][NWT];
+        int32_t t[AU][NWT];  // 0 where no relaxation is made: no candidate (>= 1) is below it
 #pragma unroll
         for (int a = 0; a < AU; a++)
 #pragma unroll
-          for (int c = 0; c < NWT; c++)
-            t[a][c] = (ok[a] && ((m[c] >> lane) & 1)) ? CT[(size_t)u[a] * S + c * 64 + lane] : 0;
+          for (int c = 0; c < NWT; c++) {
+            const bool r = ok[a] && ((m[c] >> lane) & 1) && wt[a] <= lim[c];
+            n_relax += (unsigned long long)r;
+            t[a][c] = r ? CT[(size_t)u[a] * S + c * 64 + lane] : 0;
+          }
 #pragma unroll
         for (int a = 0; a < AU; a++) {
           if (!ok[a]) continue;
@@ -237,6 +255,7 @@ __global__ __launch_bounds__(BLK) void k_cgrow(const CohortConst *__restrict__ c
       }
     }
   }
+  for (int off = 32; off > 0; off >>= 1) n_relax += __shfl_xor(n_relax, off, 64);  // per-lane counts
   if (lane == 0 && n_edges) {
     unsigned long long *st = cc.stx + (blockIdx.x & (NSTRIPE - 1)) * NSTX;
     atomicAdd(st + ST_EDGES, n_edges);

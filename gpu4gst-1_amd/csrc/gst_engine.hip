@@ -447,9 +447,11 @@ int noalloc_setup(gst_ctx *c);  // gst_noalloc.inc
 namespace {
 
 // The graph as the device holds it. With renumbering, device vertex i is the caller's
-// vertex old_of_new[i], taken in order of descending degree (ties by id), and every
-// adjacency list and group is re-sorted by device id. Otherwise the caller's arrays are
-// used as they are (adj, w, gverts point at them).
+// vertex old_of_new[i], taken in order of descending degree (ties by id), and every group
+// is re-sorted by device id. Every adjacency list is sorted by (weight, device id), with
+// or without renumbering: the grow kernels stop scanning a row at the first arc whose
+// weight lifts a state above the query's upper bound, since every later arc weighs at
+// least as much (without renumbering only gverts points at the caller's array).
 struct DeviceGraph {
   std::vector<int32_t> beg, old_of_new;
   std::vector<int32_t> adj_v, w_v, gverts_v;
@@ -471,9 +473,28 @@ DeviceGraph renumber(const gst_csr *csr, const gst_groups *groups, bool reorder)
   DeviceGraph d;
   d.beg.resize(n + 1);
   const int64_t total = groups->n_groups ? groups->offs[groups->n_groups] : 0;
+  // row of (weight, id) keys, ascending
+  auto sort_row = [](std::vector<uint64_t> &key, int32_t *adj_out, int32_t *w_out) {
+    std::sort(key.begin(), key.end());
+    for (size_t t = 0; t < key.size(); t++) {
+      w_out[t] = (int32_t)(key[t] >> 32);
+      adj_out[t] = (int32_t)(key[t] & 0xffffffffu);
+    }
+  };
   if (!reorder) {
     for (int32_t v = 0; v <= n; v++) d.beg[v] = (int32_t)csr->beg_pos[v];
-    d.adj = csr->adj; d.w = csr->w; d.gverts = groups->verts;
+    d.adj_v.resize(m);
+    d.w_v.resize(m);
+    parallel_ranges(n, [&](int64_t lo, int64_t hi) {
+      std::vector<uint64_t> key;
+      for (int64_t v = lo; v < hi; v++) {
+        const int64_t b = csr->beg_pos[v], e = csr->beg_pos[v + 1];
+        key.resize(e - b);
+        for (int64_t t = b; t < e; t++) key[t - b] = ((uint64_t)(uint32_t)csr->w[t] << 32) | (uint32_t)csr->adj[t];
+        sort_row(key, d.adj_v.data() + b, d.w_v.data() + b);
+      }
+    });
+    d.gverts = groups->verts;
     return d;
   }
   d.old_of_new.resize(n);
@@ -494,12 +515,8 @@ DeviceGraph renumber(const gst_csr *csr, const gst_groups *groups, bool reorder)
       const int64_t b = csr->beg_pos[o], e = csr->beg_pos[o + 1];
       key.resize(e - b);
       for (int64_t t = b; t < e; t++)
-        key[t - b] = ((uint64_t)(uint32_t)new_of_old[csr->adj[t]] << 32) | (uint32_t)csr->w[t];
-      std::sort(key.begin(), key.end());
-      for (int64_t t = 0; t < e - b; t++) {
-        d.adj_v[d.beg[i] + t] = (int32_t)(key[t] >> 32);
-        d.w_v[d.beg[i] + t] = (int32_t)(key[t] & 0xffffffffu);
-      }
+        key[t - b] = ((uint64_t)(uint32_t)csr->w[t] << 32) | (uint32_t)new_of_old[csr->adj[t]];
+      sort_row(key, d.adj_v.data() + d.beg[i], d.w_v.data() + d.beg[i]);
     }
   });
   d.gverts_v.resize(total);

@@ -948,19 +948,29 @@ __global__ __launch_bounds__(BLK) void k_farsplit(const JobConst *jcs, JobState 
 // skipped when the bits are already set.
 // dbase/dstride: target u's level row is dbase + u * dstride. Edges with h[j] false are
 // skipped (want[j] = false, nch[j] = 0).
+// ub: the job's upper bound (INF when it does not prune). A relaxation whose result
+// exceeds it is not made: every state of an optimal derivation costs at most OPT <= ub,
+// and a state above ub can neither lower ub (every candidate built on it costs more) nor
+// become a merge half of a state <= ub, so its target is not even read. n_rel counts the
+// relaxations made (target loads).
 template <int NE>
 __device__ __forceinline__ void relax_edges(int32_t *dbase, size_t dstride, unsigned long long *act_lo,
                                             const int32_t *nchk, int32_t nheavy, const RelaxGroup &G, const bool (&h)[NE],
                                             const int32_t (&u)[NE], const int32_t (&w)[NE], bool (&want)[NE],
-                                            int (&nch)[NE], int &n_atom) {
+                                            int (&nch)[NE], int &n_atom, int ub, int &n_rel) {
   gi32 *Td[NE];
   gu64 *const act = as_global(act_lo);
-  int t[NE][RG];
+  int t[NE][RG];  // 0 where no relaxation is made: no candidate (>= 1) is below it
 #pragma unroll
   for (int j = 0; j < NE; j++) {
     Td[j] = as_global(dbase + (size_t)u[j] * dstride);
+    const int lj = h[j] ? ub - w[j] : -1;  // the largest source cost this arc relaxes
 #pragma unroll
-    for (int i = 0; i < RG; i++) t[j][i] = (h[j] && i < G.n) ? Td[j][G.p[i]] : 0;
+    for (int i = 0; i < RG; i++) {
+      const bool r = i < G.n && G.c[i] <= lj;
+      n_rel += (int)r;
+      t[j][i] = r ? Td[j][G.p[i]] : 0;
+    }
   }
   unsigned long long imp[NE];
 #pragma unroll
@@ -993,6 +1003,35 @@ __device__ __forceinline__ void relax_edges(int32_t *dbase, size_t dstride, unsi
   for (int j = 0; j < NE; j++) nch[j] = want[j] ? chunks_of(nchk, nheavy, u[j]) : 0;
 }
 
+// Arcs of the row [e0, e0 + deg) with weight <= lim: rows are sorted by weight, so they
+// are a prefix, found by binary search (first and last weight checked first).
+__device__ __forceinline__ int arcs_within(const int32_t *wts, int32_t e0, int deg, int lim) {
+  if (deg <= 0) return 0;
+  const gi32 *w = as_global(wts) + e0;
+  const int wf = w[0], wl = w[deg - 1];
+  if (wl <= lim) return deg;
+  if (wf > lim) return 0;
+  int lo = 1, hi = deg - 1;  // w[lo - 1] <= lim < w[hi]
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (w[mid] <= lim) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+// Upper bound a job's grow kernels cut relaxations at (INF when it does not prune).
+__device__ __forceinline__ int relax_ub(const JobConst &c, const JobState *s) {
+  return (c.flags & JF_NO_PRUNE) ? INF : load_ub(s);
+}
+
+// Smallest source cost of a relax group.
+__device__ __forceinline__ int group_min(const RelaxGroup &G) {
+  int m = INF;
+#pragma unroll
+  for (int i = 0; i < RG; i++) m = (i < G.n && G.c[i] < m) ? G.c[i] : m;
+  return m;
+}
+
 // Thread items (degree < split), edge-balanced. A wavefront takes 64 consecutive items and
 // spreads their arcs over its lanes: a wave-level prefix sum of the item degrees gives
 // each item's first arc slot, and a lane finds the item of its slot by binary search in
@@ -1006,6 +1045,7 @@ struct ThreadSeg {
   int32_t jb[WAVE], lo[WAVE];
   int32_t gn[WAVE];       // states in this pass's group
   int32_t gp[WAVE][RG], gc[WAVE][RG];
+  int32_t ub[WAVE];       // the item's job's upper bound
   int32_t *dbase[WAVE];
   unsigned long long *act[WAVE];
   long long dstride[WAVE];
@@ -1013,7 +1053,7 @@ struct ThreadSeg {
   int32_t nhv[WAVE];
 };
 
-__global__ __launch_bounds__(BLK) void k_grow_thread(const JobConst *__restrict__ jcs, JobState *jss, const Plan *pl, GraphDev g) {
+__global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) void k_grow_thread(const JobConst *__restrict__ jcs, JobState *jss, const Plan *pl, GraphDev g) {
   __shared__ long long soff[MAXJOBS + 1];
   __shared__ int32_t pb_small[(BLK / WAVE) * PB_SMALL];
   __shared__ int32_t pb_big[(BLK / WAVE) * 2 * PB_BIG];
@@ -1041,7 +1081,8 @@ __global__ __launch_bounds__(BLK) void k_grow_thread(const JobConst *__restrict_
   unsigned long long *uanxt = nullptr;
   for (long long wbase = (long long)blockIdx.x * BLK + (threadIdx.x & ~63); wbase < total; wbase += stride) {
     const long long it = wbase + lane;
-    int deg = 0;
+    int deg = 0, iub = INF;
+    int32_t e0 = 0;
     const int32_t *src = nullptr;
     unsigned long long gm = 0;
     const int jf = __builtin_amdgcn_readfirstlane(find_job(soff, nj, wbase));
@@ -1096,24 +1137,22 @@ __global__ __launch_bounds__(BLK) void k_grow_thread(const JobConst *__restrict_
       }
       if (jb != cj) { flush_lane(jcs, jss, cj, k); k.zero(); cj = jb; }
       k.verts++;
-      const int32_t e0 = g.beg[v];
+      e0 = g.beg[v];
       deg = g.beg[v + 1] - e0;
-      if (gm && deg) {
-        const int nb = popc((uint32_t)gm) + popc((uint32_t)(gm >> 32));
-        k.edges += deg;
-        k.relax += deg * nb;
-      } else {
-        gm = 0;
-      }
+      iub = relax_ub(jcs[jb], &jss[jb]);
+      if (!deg) gm = 0;
       S.e0[lane] = e0;
       S.jb[lane] = jb;
       S.lo[lane] = lo;
+      S.ub[lane] = iub;
     }
     for (;;) {  // one pass per group of up to RG active states of every item
       RelaxGroup G;
       G.n = 0;
       if (gm) take_group(src, gm, G);
-      const int cnt = G.n ? deg : 0;
+      // only the row's prefix of arcs light enough for some state of the group
+      const int cnt = G.n ? arcs_within(wts, e0, deg, iub - group_min(G)) : 0;
+      k.edges += cnt;
       if (!__any(cnt > 0)) break;
       S.gn[lane] = G.n;
 #pragma unroll
@@ -1148,7 +1187,8 @@ __global__ __launch_bounds__(BLK) void k_grow_thread(const JobConst *__restrict_
           Gj.n = S.gn[j];
 #pragma unroll
           for (int i = 0; i < RG; i++) { Gj.p[i] = S.gp[j][i]; Gj.c[i] = S.gc[j][i]; }
-          relax_edges<1>(S.dbase[j], (size_t)S.dstride[j], S.act[j], S.nck[j], S.nhv[j], Gj, h, u, w, want, nch, k.atom);
+          relax_edges<1>(S.dbase[j], (size_t)S.dstride[j], S.act[j], S.nck[j], S.nhv[j], Gj, h, u, w, want, nch, k.atom,
+                         S.ub[j], k.relax);
           ljb = S.jb[j];
           llo = S.lo[j];
         }
@@ -1167,7 +1207,7 @@ __global__ __launch_bounds__(BLK) void k_grow_thread(const JobConst *__restrict_
 // together: most arcs of this kernel sit in chunks of 512-1024 edges, which then take
 // 2-4 dependent round-trip chains instead of 4-8.
 constexpr int GW_NE = 4;
-__global__ __launch_bounds__(BLK) void k_grow_wave(const JobConst *__restrict__ jcs, JobState *jss, const Plan *pl, GraphDev g) {
+__global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) void k_grow_wave(const JobConst *__restrict__ jcs, JobState *jss, const Plan *pl, GraphDev g) {
   __shared__ long long soff[MAXJOBS + 1];
   __shared__ int32_t pb_small[(BLK / WAVE) * PB_SMALL];
   __shared__ int32_t pb_big[(BLK / WAVE) * 2 * PB_BIG];
@@ -1184,7 +1224,7 @@ __global__ __launch_bounds__(BLK) void k_grow_wave(const JobConst *__restrict__ 
   const long long waves_per_grid = (long long)gridDim.x * (BLK / WAVE);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: scalar registers
   // fields of job cj (set when cj changes)
-  int jlo = 0, jL = 0, jd = 0, jheavy = 0;
+  int jlo = 0, jL = 0, jd = 0, jheavy = 0, jub = INF;
   long long jrow = 0;
   int32_t *jsrc = nullptr, *jdst = nullptr;
   const int32_t *jnck = nullptr;
@@ -1228,6 +1268,9 @@ __global__ __launch_bounds__(BLK) void k_grow_wave(const JobConst *__restrict__ 
       jact = c.act + (size_t)jlo * g.n;
       jnck = c.nchunk;
       jheavy = __builtin_amdgcn_readfirstlane(c.n_heavy);
+      // the job's upper bound as the wave first meets the job: it only decreases, so a
+      // value read earlier in the launch still bounds every optimal derivation
+      jub = __builtin_amdgcn_readfirstlane(relax_ub(c, s));
     }
     const int lo = jlo;
     const size_t dstride = (size_t)jrow;
@@ -1239,17 +1282,18 @@ __global__ __launch_bounds__(BLK) void k_grow_wave(const JobConst *__restrict__ 
     const int32_t *src = jsrc + (size_t)v * dstride;
     if (lane == 0 && ch == 0) k.verts++;
     if (!gm) continue;
-    const int nb = popc((uint32_t)gm) + popc((uint32_t)(gm >> 32));
     const int32_t eb = __builtin_amdgcn_readlane(heb, l);
     const int32_t ee = __builtin_amdgcn_readlane(hee, l);
-    if (lane == 0) {
-      k.edges += ee - eb;
-      k.relax += (ee - eb) * nb;
-    }
+    const int ub = jub;
+    const int w0 = wts[eb];  // the chunk's lightest arc (eb < ee: chunks are never empty)
     while (gm) {
       RelaxGroup G;
       take_group(src, gm, G);
       uniform_group(G);
+      // the row is sorted by weight: arcs heavier than lim lift every state of the group
+      // above the upper bound, and so does the rest of the row after the first of them
+      const int lim = ub - group_min(G);
+      if (w0 > lim) continue;
       for (int32_t e = eb; e < ee; e += GW_NE * WAVE) {  // uniform trip count
         bool h[GW_NE], want[GW_NE];
         int32_t u[GW_NE], w[GW_NE];
@@ -1261,7 +1305,9 @@ __global__ __launch_bounds__(BLK) void k_grow_wave(const JobConst *__restrict__ 
           u[j] = h[j] ? adj[ej] : 0;
           w[j] = h[j] ? wts[ej] : 0;
         }
-        relax_edges<GW_NE>(dbase, dstride, act_lo, jnck, jheavy, G, h, u, w, want, nch, k.atom);
+        if (__builtin_amdgcn_readfirstlane(w[0]) > lim) break;  // lane 0 holds arc e
+        if (lane == 0) k.edges += min(GW_NE * WAVE, ee - e);
+        relax_edges<GW_NE>(dbase, dstride, act_lo, jnck, jheavy, G, h, u, w, want, nch, k.atom, ub, k.relax);
 #pragma unroll
         for (int j = 0; j < GW_NE; j++) {
           wp_push(pb, jcs, jss, g, jb, lo, want[j], u[j], nch[j]);
