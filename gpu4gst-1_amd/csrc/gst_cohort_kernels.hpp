@@ -195,12 +195,21 @@ __global__ __launch_bounds__(BLK) void k_cgrow(const CohortConst *__restrict__ c
       nb += __popcll(x);
     }
     if (!nb) continue;
-    // rows are sorted by weight: the scan ends at the first arc heavier than every lim
-    for (int off = 32; off > 0; off >>= 1) {
-      const int o = __shfl_xor(lmax, off, 64);
-      lmax = o > lmax ? o : lmax;
+    // rows are sorted by weight: the scan ends at the first arc heavier than every lim,
+    // and a chunk is skipped for arcs heavier than all of its lims (cmx: per chunk)
+    int32_t cmx[NWT];
+#pragma unroll
+    for (int c = 0; c < NWT; c++) {
+      int x = lim[c];
+      for (int off = 32; off > 0; off >>= 1) {
+        const int o = __shfl_xor(x, off, 64);
+        x = o > x ? o : x;
+      }
+      cmx[c] = __builtin_amdgcn_readfirstlane(x);
     }
-    lmax = __builtin_amdgcn_readfirstlane(lmax);
+    lmax = -1;
+#pragma unroll
+    for (int c = 0; c < NWT; c++) lmax = cmx[c] > lmax ? cmx[c] : lmax;
     const int32_t vb = g.beg[v], ve = g.beg[v + 1];
     const int32_t eb = ch < 0 ? vb : vb + ch * cc.chunk;
     const int32_t ee = ch < 0 ? ve : min(ve, eb + cc.chunk);
@@ -224,9 +233,12 @@ __global__ __launch_bounds__(BLK) void k_cgrow(const CohortConst *__restrict__ c
         for (int a = 0; a < AU; a++)
 #pragma unroll
           for (int c = 0; c < NWT; c++) {
-            const bool r = ok[a] && ((m[c] >> lane) & 1) && wt[a] <= lim[c];
-            n_relax += (unsigned long long)r;
-            t[a][c] = r ? CT[(size_t)u[a] * S + c * 64 + lane] : 0;
+            t[a][c] = 0;
+            if (ok[a] && wt[a] <= cmx[c]) {  // uniform
+              const bool r = wt[a] <= lim[c];  // lim is -1 for labels not grown
+              n_relax += (unsigned long long)r;
+              if (r) t[a][c] = CT[(size_t)u[a] * S + c * 64 + lane];
+            }
           }
 #pragma unroll
         for (int a = 0; a < AU; a++) {
@@ -235,14 +247,15 @@ __global__ __launch_bounds__(BLK) void k_cgrow(const CohortConst *__restrict__ c
           bool any = false;
 #pragma unroll
           for (int c = 0; c < NWT; c++) {
-            if (!m[c]) continue;  // uniform
+            if (wt[a] > cmx[c]) continue;  // uniform (also every chunk without grown labels)
             int nc = sv[c] + wt[a];
             nc = nc < INF ? nc : INF;
             const bool imp = ((m[c] >> lane) & 1) && nc < t[a][c];
             if (imp)
               __hip_atomic_fetch_min(&CT[(size_t)u[a] * S + c * 64 + lane], nc, __ATOMIC_RELAXED,
                                      __HIP_MEMORY_SCOPE_AGENT);
-            const unsigned long long im = __ballot(imp);
+            // activated only where the select pass would grow it (<= the member's UB/2)
+            const unsigned long long im = __ballot(imp && nc <= cap[c]);
             if (lane == c) imc = im;
             any = any || im != 0;
           }

@@ -953,11 +953,14 @@ __global__ __launch_bounds__(BLK) void k_farsplit(const JobConst *jcs, JobState 
 // and a state above ub can neither lower ub (every candidate built on it costs more) nor
 // become a merge half of a state <= ub, so its target is not even read. n_rel counts the
 // relaxations made (target loads).
+// acap: the largest cost an improved target is activated at (act_cap); a state above it
+// is stored, as a merge input, but the select pass would drop it unevaluated for growth,
+// so it is not pushed.
 template <int NE>
 __device__ __forceinline__ void relax_edges(int32_t *dbase, size_t dstride, unsigned long long *act_lo,
                                             const int32_t *nchk, int32_t nheavy, const RelaxGroup &G, const bool (&h)[NE],
                                             const int32_t (&u)[NE], const int32_t (&w)[NE], bool (&want)[NE],
-                                            int (&nch)[NE], int &n_atom, int ub, int &n_rel) {
+                                            int (&nch)[NE], int &n_atom, int ub, int acap, int &n_rel) {
   gi32 *Td[NE];
   gu64 *const act = as_global(act_lo);
   int t[NE][RG];  // 0 where no relaxation is made: no candidate (>= 1) is below it
@@ -984,7 +987,7 @@ __device__ __forceinline__ void relax_edges(int32_t *dbase, size_t dstride, unsi
         if (nc < t[j][i]) {
           __hip_atomic_fetch_min(&Td[j][G.p[i]], nc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           n_atom++;
-          imp[j] |= 1ull << G.p[i];
+          if (nc <= acap) imp[j] |= 1ull << G.p[i];
         }
       }
     }
@@ -1024,6 +1027,13 @@ __device__ __forceinline__ int relax_ub(const JobConst &c, const JobState *s) {
   return (c.flags & JF_NO_PRUNE) ? INF : load_ub(s);
 }
 
+// Largest cost at which a grown state is activated: non-hop states above floor(UB/2) are
+// never grown (the centroid rule of the select pass); diameter-bounded ones are grown up
+// to UB (their select pass has no UB/2 rule).
+__device__ __forceinline__ int act_cap(const JobConst &c, int ub) {
+  return (c.hop || ub >= INF) ? ub : ub >> 1;
+}
+
 // Smallest source cost of a relax group.
 __device__ __forceinline__ int group_min(const RelaxGroup &G) {
   int m = INF;
@@ -1046,9 +1056,10 @@ struct ThreadSeg {
   int32_t gn[WAVE];       // states in this pass's group
   int32_t gp[WAVE][RG], gc[WAVE][RG];
   int32_t ub[WAVE];       // the item's job's upper bound
+  int32_t acap[WAVE];     // and its activation cap
   int32_t *dbase[WAVE];
   unsigned long long *act[WAVE];
-  long long dstride[WAVE];
+  int32_t dstride[WAVE];  // row stride H * C(k, L) <= 16 * 35
   const int32_t *nck[WAVE];  // the item's job: chunk counts and n_heavy
   int32_t nhv[WAVE];
 };
@@ -1145,6 +1156,7 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) void k
       S.jb[lane] = jb;
       S.lo[lane] = lo;
       S.ub[lane] = iub;
+      S.acap[lane] = act_cap(jcs[jb], iub);
     }
     for (;;) {  // one pass per group of up to RG active states of every item
       RelaxGroup G;
@@ -1188,7 +1200,7 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) void k
 #pragma unroll
           for (int i = 0; i < RG; i++) { Gj.p[i] = S.gp[j][i]; Gj.c[i] = S.gc[j][i]; }
           relax_edges<1>(S.dbase[j], (size_t)S.dstride[j], S.act[j], S.nck[j], S.nhv[j], Gj, h, u, w, want, nch, k.atom,
-                         S.ub[j], k.relax);
+                         S.ub[j], S.acap[j], k.relax);
           ljb = S.jb[j];
           llo = S.lo[j];
         }
@@ -1224,7 +1236,7 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) void k
   const long long waves_per_grid = (long long)gridDim.x * (BLK / WAVE);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: scalar registers
   // fields of job cj (set when cj changes)
-  int jlo = 0, jL = 0, jd = 0, jheavy = 0, jub = INF;
+  int jlo = 0, jL = 0, jd = 0, jheavy = 0, jub = INF, jacap = INF;
   long long jrow = 0;
   int32_t *jsrc = nullptr, *jdst = nullptr;
   const int32_t *jnck = nullptr;
@@ -1271,6 +1283,7 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) void k
       // the job's upper bound as the wave first meets the job: it only decreases, so a
       // value read earlier in the launch still bounds every optimal derivation
       jub = __builtin_amdgcn_readfirstlane(relax_ub(c, s));
+      jacap = act_cap(c, jub);
     }
     const int lo = jlo;
     const size_t dstride = (size_t)jrow;
@@ -1307,7 +1320,7 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) void k
         }
         if (__builtin_amdgcn_readfirstlane(w[0]) > lim) break;  // lane 0 holds arc e
         if (lane == 0) k.edges += min(GW_NE * WAVE, ee - e);
-        relax_edges<GW_NE>(dbase, dstride, act_lo, jnck, jheavy, G, h, u, w, want, nch, k.atom, ub, k.relax);
+        relax_edges<GW_NE>(dbase, dstride, act_lo, jnck, jheavy, G, h, u, w, want, nch, k.atom, ub, jacap, k.relax);
 #pragma unroll
         for (int j = 0; j < GW_NE; j++) {
           wp_push(pb, jcs, jss, g, jb, lo, want[j], u[j], nch[j]);
