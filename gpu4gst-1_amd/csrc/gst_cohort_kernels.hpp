@@ -184,7 +184,7 @@ __global__ __launch_bounds__(BLK) void k_cgrow(const CohortConst *__restrict__ c
         const int q = (c * 64 + lane) / cc.C;
         if (!(jcs[cc.jobs[q]].flags & JF_NO_PRUNE)) {
           ubq = load_ub(&jss[cc.jobs[q]]);
-          cap[c] = ubq >> 1;
+          cap[c] = grow_cap(ubq, cc.L);
         }
       }
       const bool grow = bit && sv[c] <= cap[c];

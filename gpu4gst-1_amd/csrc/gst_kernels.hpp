@@ -206,6 +206,21 @@ __device__ __forceinline__ int load_ub(const JobState *s) {
   return __hip_atomic_load(&s->ub, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+#ifndef GST_L1_FULL
+#define GST_L1_FULL 1
+#endif
+// Largest cost at which a non-hop state of level L is grown: floor(UB/2), the centroid
+// rule, at levels >= 2; level 1 grows up to UB - 1, so that its distances are exact up to
+// UB (a shortest path's last source costs at most its length - 1) and the lower bounds
+// built on them (lb_tail, the tour tables) are clamped at UB + 1 instead of UB/2 + 1.
+// Growing more at level 1 is always safe (it only makes more values exact); on the C4
+// workload it costs 15% more level-1 relaxations and removes 90% of those of levels 2-4.
+// GST_L1_FULL=0 restores the centroid rule at level 1 as well.
+__device__ __forceinline__ int grow_cap(int ub, int L) {
+  if (GST_L1_FULL && L == 1) return ub >= INF ? ub : ub - 1;
+  return ub >> 1;
+}
+
 // list index of the worklist a job is reading (cur) / writing (nxt) this round
 __device__ __forceinline__ int cur_list(const JobConst &c, const JobState *s) {
   return c.hop ? s->depth : s->parity;
@@ -519,7 +534,7 @@ __device__ __forceinline__ unsigned long long grow_select_rows(const JobConst &c
       ubl = cand;
     }
     if (prune) {
-      if (cst > (ubl >> 1)) continue;
+      if (cst > grow_cap(ubl, L)) continue;
       if (L > 1 && cst + lb_tail(sl, r1, p, c, hk, w2) > ubl) continue;
     }
     if (cst > theta) {  // near-far: wait in the far pile
@@ -1030,8 +1045,8 @@ __device__ __forceinline__ int relax_ub(const JobConst &c, const JobState *s) {
 // Largest cost at which a grown state is activated: non-hop states above floor(UB/2) are
 // never grown (the centroid rule of the select pass); diameter-bounded ones are grown up
 // to UB (their select pass has no UB/2 rule).
-__device__ __forceinline__ int act_cap(const JobConst &c, int ub) {
-  return (c.hop || ub >= INF) ? ub : ub >> 1;
+__device__ __forceinline__ int act_cap(const JobConst &c, int ub, int L) {
+  return (c.hop || ub >= INF) ? ub : grow_cap(ub, L);
 }
 
 // Smallest source cost of a relax group.
@@ -1156,7 +1171,7 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) void k
       S.jb[lane] = jb;
       S.lo[lane] = lo;
       S.ub[lane] = iub;
-      S.acap[lane] = act_cap(jcs[jb], iub);
+      S.acap[lane] = act_cap(jcs[jb], iub, jss[jb].level);
     }
     for (;;) {  // one pass per group of up to RG active states of every item
       RelaxGroup G;
@@ -1283,7 +1298,7 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) void k
       // the job's upper bound as the wave first meets the job: it only decreases, so a
       // value read earlier in the launch still bounds every optimal derivation
       jub = __builtin_amdgcn_readfirstlane(relax_ub(c, s));
-      jacap = act_cap(c, jub);
+      jacap = act_cap(c, jub, jL);
     }
     const int lo = jlo;
     const size_t dstride = (size_t)jrow;
@@ -1376,7 +1391,8 @@ __global__ __launch_bounds__(BLK) void k_tables(const JobConst *__restrict__ jcs
   const int K = c.K, W = c.W;
   if (threadIdx.x == 0) {
     const int ub = load_ub(s);
-    long long r = (long long)(ub >> 1) + 1;
+    // level-1 distances are exact up to grow_cap(UB, 1) + 1
+    long long r = c.hop ? (long long)(ub >> 1) + 1 : (long long)grow_cap(ub, 1) + 1;
     r1s = (int)(r < LB_CAP ? r : LB_CAP);
     s->r1 = r1s;
   }
