@@ -104,6 +104,8 @@ struct gst_ctx {
   std::vector<int32_t> comp;        // connected component of each vertex (caller's ids)
   std::vector<int32_t> h_deg;       // degrees in device order
   std::vector<int32_t> old_of_new;  // device vertex -> caller's id (empty: same numbering)
+  std::vector<int32_t> hub_gd;      // [n_hubs][n_groups] hub-to-group distances (gst_hubs.inc)
+  int n_hubs = 0;
   int32_t n_groups = 0;
   int64_t graph_bytes = 0;
   long long bcap = 0;  // wavefront items one worklist can hold
@@ -235,7 +237,7 @@ int check_query(const gst_ctx *c, const int32_t *gids, int32_t k, int32_t D, int
   }
   if (D > GST_MAX_D) { set_err("D > GST_MAX_D"); return GST_BAD_INPUT; }
   q.D = D < 0 ? -1 : D;
-  q.flags = flags | (c->opts.flags & ~GST_O_KEEP_ORDER);
+  q.flags = flags | (c->opts.flags & ~(GST_O_KEEP_ORDER | GST_O_NO_HUB_INDEX));
   return GST_OK;
 }
 
@@ -437,6 +439,7 @@ void finish_result(const gst_ctx *c, const Query &q, const JobState &js, int32_t
 
 }  // namespace
 
+#include "gst_hubs.inc"
 #include "gst_batch.inc"
 #include "gst_cohort.inc"
 
@@ -677,6 +680,10 @@ int gst_open(const gst_csr *csr, const gst_groups *groups, const gst_opts *opts,
   }
   int rc = gst_set_tuning(c.get(), split0, chunk0, c->delta);
   if (rc) return rc;
+  if (!(c->opts.flags & GST_O_NO_HUB_INDEX)) {
+    rc = hub_index_build(c.get());
+    if (rc) return rc;
+  }
   rc = batch_open(c.get());
   if (rc) return rc;
   rc = noalloc_setup(c.get());
@@ -811,7 +818,7 @@ static int solve_on_slot(gst_ctx *c, Slot &s, const Query &q, int32_t *ebuf, int
   JobConst &jc = *c->h_jc;
   fill_jobconst(c, s, q, jc);
   JobState &js = c->h_js[0];
-  init_jobstate(js);
+  init_job(c, q, js);
   HIPCHK(hipMemcpyAsync(c->d_jc, &jc, sizeof(jc), hipMemcpyHostToDevice, c->st));
   HIPCHK(hipMemcpyAsync(c->d_js, &js, sizeof(js), hipMemcpyHostToDevice, c->st));
   launch_job_init(c->st, s, c, jc);

@@ -25,6 +25,7 @@ GST_COST_RANGE = -6
 COST_CEILING = 0x3fffffff
 F_STATS, F_NO_TREE, F_NO_PRUNE = 1, 2, 4
 O_KEEP_ORDER = 0x100  # gst_opts.flags: keep the caller's vertex numbering on the device
+O_NO_HUB_INDEX = 0x200  # gst_opts.flags: no hub distance index (queries start at UB = INF)
 MAX_K, MAX_D = 7, 15
 
 

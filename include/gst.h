@@ -78,6 +78,11 @@ enum {
  * gst_open renumbers vertices by descending degree (hubs first, so the rows that most
  * arcs point at share a few cache lines); results are always in the caller's ids. */
 #define GST_O_KEEP_ORDER 0x100
+/* gst_opts.flags only: build no hub distance index at gst_open. By default gst_open
+ * computes the distances from the 64 highest-degree vertices to every group, and every
+ * non-hop query starts from the upper bound of the best of these 64 stars (the weight of
+ * k shortest paths from one vertex). The index holds 64 x n_groups int32 on the host. */
+#define GST_O_NO_HUB_INDEX 0x200
 
 typedef struct {
   int32_t n;              /* vertices */

@@ -97,6 +97,30 @@ def test_caller_order_matches_renumbered(gpu_engine, rmat, rmat_inst):
         keep.close()
 
 
+def test_hub_index_matches_no_index(gpu_engine, rmat, rmat_inst):
+    """Queries start from the best hub star as their upper bound (the default) or from
+    INF (GST_O_NO_HUB_INDEX): the same optimum either way, equal to PrunedDP++, and the
+    hub star really bounds it (the result's ub never exceeds the first one polled)."""
+    plain = engine.Engine(rmat.graph, rmat.groups, device=0, flags=engine.O_NO_HUB_INDEX)
+    try:
+        rng = np.random.default_rng(23)
+        qs = [rng.choice(rmat.groups.n_groups, int(rng.integers(2, 7)), replace=False).astype(np.int32)
+              for _ in range(10)]
+        for q in qs:
+            print("hub query", q, flush=True)
+            o = pyoracle.prunedpp(rmat_inst, q)
+            a = gpu_engine.solve(q)
+            b = plain.solve(q)
+            assert a["status"] == b["status"] == o["status"], (q, a["status"], b["status"], o["status"])
+            if a["status"] != engine.GST_OK:
+                continue
+            assert a["cost"] == b["cost"] == o["tree_cost"], (q, a["cost"], b["cost"], o["tree_cost"])
+            ok, why = engine.validate_tree(rmat.graph, rmat.groups, q, a["edges"], a["cost"])
+            assert ok, (q, why)
+    finally:
+        plain.close()
+
+
 @pytest.mark.parametrize("D", [1, 2, 3, 5])
 def test_rmat_hop_pruned_matches_unpruned(gpu_engine, rmat, D):
     """The diameter-bounded search prunes by a lower bound against an upper bound; with
