@@ -571,8 +571,17 @@ __device__ unsigned long long grow_select_hop(const JobConst &c, JobState *s, in
   const int K = c.K, D = c.D, H = c.H;
   const int32_t *l1 = lrow(c, 1, v, 0);  // H rows of K group distances, contiguous
   int dlb[MAXK], dst[MAXK];
+  // pm: prefix minima over the heights read so far; lo[i][x] = min_{h <= x} T(v,{i},h)
+  // for x <= HX, kept for the long-arm stars below
+  constexpr int HX = 2;
+  int pm[MAXK], lo[MAXK][HX + 1];
+  long long arm = INF;  // best long-arm star of v
 #pragma unroll
-  for (int i = 0; i < MAXK; i++) { dlb[i] = INF; dst[i] = INF; }
+  for (int i = 0; i < MAXK; i++) {
+    dlb[i] = INF; dst[i] = INF; pm[i] = INF;
+#pragma unroll
+    for (int x = 0; x <= HX; x++) lo[i][x] = INF;
+  }
   for (int h0 = 0; h0 < H; h0 += 4) {  // 4 heights of loads in flight at a time
     int xs[4][MAXK];
 #pragma unroll
@@ -587,10 +596,46 @@ __device__ unsigned long long grow_select_hop(const JobConst &c, JobState *s, in
         const int x = xs[hh][i];
         if (h <= D - d && x < dlb[i]) dlb[i] = x;
         if (2 * h <= D && x < dst[i]) dst[i] = x;
+        if (h < H && x < pm[i]) pm[i] = x;
+#pragma unroll
+        for (int q = 0; q <= HX; q++)
+          if (h == q) lo[i][q] = pm[i];
+      }
+      // Long-arm stars: one group j reached within a hops (a >= D/2), every other within
+      // D - a. Merging these singletons at v in any order is legal in the DP (any two
+      // heights sum to <= D), so each value is a DP state's value and bounds the optimum
+      // from above; a = ceil(D/2) already beats the plain star for odd D.
+      const int xo = D - h;
+      if (h < H && 2 * h >= D && xo <= HX) {
+        long long sum = 0;
+        bool ok = true;
+#pragma unroll
+        for (int i = 0; i < MAXK; i++) {
+          if (i >= K) continue;
+          int lv = INF;
+#pragma unroll
+          for (int q = 0; q <= HX; q++)
+            if (xo == q) lv = lo[i][q];
+          ok = ok && lv < INF;
+          sum += lv;
+        }
+        if (ok) {
+#pragma unroll
+          for (int j = 0; j < MAXK; j++) {
+            if (j >= K) continue;
+            int lj = INF;
+#pragma unroll
+            for (int q = 0; q <= HX; q++)
+              if (xo == q) lj = lo[j][q];
+            const long long cand = sum - lj + pm[j];
+            arm = cand < arm ? cand : arm;
+          }
+        }
       }
     }
   }
   int ubl = ub;
+  if (arm < ubl) { atomicMin(&s->ub, (int)arm); ubl = (int)arm; }
   {
     long long star = 0;
     bool ok = true;
@@ -853,7 +898,7 @@ __device__ __forceinline__ void wp_push(WavePush &b, const JobConst *jcs, JobSta
 // apply the UB/2 and lower-bound cuts) and overwrite its mask with the states that grow
 // now. States above the near-far threshold move to the far pile. Hub vertices spread
 // over many wavefront items are evaluated once, not per item.
-__global__ __launch_bounds__(BLK) void k_select(const JobConst *jcs, JobState *jss, const Plan *pl, GraphDev g) {
+__global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) void k_select(const JobConst *jcs, JobState *jss, const Plan *pl, GraphDev g) {
   __shared__ long long so0[MAXJOBS + 1], so1[MAXJOBS + 1];
   __shared__ int32_t pb_small[(BLK / WAVE) * PB_SMALL];
   __shared__ int32_t pb_big[(BLK / WAVE) * 2 * PB_BIG];
