@@ -278,6 +278,35 @@ __global__ __launch_bounds__(BLK) void k_cgrow(const CohortConst *__restrict__ c
   }
 }
 
+// Level-1 stars of every member over every vertex, after the cohort's level-1 rounds (the
+// UB candidates a level-1 select pass would offer): a wavefront per vertex, lane q =
+// member q, sum_i T_q(v, {i}) when every label is finite; each lane keeps its minimum and
+// lowers its member's UB once at the end. A label within reach of INF flags its member's
+// saturation check, as the select pass does.
+__global__ __launch_bounds__(BLK) void k_cstar(const CohortConst *__restrict__ ccp, const JobConst *jcs, JobState *jss,
+                                               GraphDev g) {
+  const CohortConst &cc = *ccp;
+  const int lane = lane_id();
+  long long best = INF;
+  bool sat = false;
+  if (lane < cc.B) {
+    const gi32 *CT = as_global(cc.CT);
+    for (long long v = (long long)blockIdx.x * (BLK / WAVE) + (threadIdx.x >> 6); v < g.n;
+         v += (long long)gridDim.x * (BLK / WAVE)) {
+      long long sum = 0;
+      for (int i = 0; i < cc.K; i++) {
+        const int x = CT[(size_t)v * cc.S + lane * cc.C + i];
+        sat = sat || (x >= SAT_AT && x < INF);
+        sum = x >= INF ? (long long)INF : sum + x;
+      }
+      best = sum < best ? sum : best;
+    }
+    JobState *s = &jss[cc.jobs[lane]];
+    if (best < INF && !(jcs[cc.jobs[lane]].flags & JF_NO_PRUNE)) atomicMin(&s->ub, (int)best);
+    if (sat) flag_saturation(s);
+  }
+}
+
 // Zero the label masks of the list just consumed (every item of a vertex read them).
 __global__ __launch_bounds__(BLK) void k_cclear(const CohortConst *__restrict__ ccp, const CohortState *__restrict__ csp,
                                                 GraphDev g) {
