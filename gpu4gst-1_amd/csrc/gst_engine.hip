@@ -221,6 +221,11 @@ int plain_slot(gst_ctx *c, int K, int D) {
 }
 
 // Validated, de-duplicated query (fixed capacity: no heap use).
+// Non-hop pruned merges of levels >= 2 over the vertices k_alive lists (0: every vertex).
+#ifndef GST_SPARSE_MERGE
+#define GST_SPARSE_MERGE 1
+#endif
+
 struct Query {
   int32_t gids[MAXK] = {0};
   int K = 0, D = -1, flags = 0;
@@ -248,6 +253,10 @@ void fill_jobconst(const gst_ctx *c, const Slot &s, const Query &q, JobConst &jc
   jc.bcap = c->bcap; jc.scap = SCAP; jc.ecap = s.ecap;
   jc.K = q.K; jc.D = q.D; jc.H = q.D >= 0 ? q.D + 1 : 1; jc.W = 1 << q.K; jc.R = jc.W - 1;
   jc.hop = q.D >= 0; jc.flags = q.flags; jc.nl = s.nlst;
+  // non-hop pruned merges of levels >= 2 cover only the vertices k_alive lists, kept in
+  // the arena's unused tail (levels 1..k take n * (W - 1) of its n * W words)
+  jc.sparse = GST_SPARSE_MERGE && q.D < 0 && !(q.flags & GST_F_NO_PRUNE);
+  jc.alv = s.T + (size_t)c->n * (jc.W - 1);
   jc.delta = q.D >= 0 ? 0 : c->delta;
   {
     const gst_ctx::TuneSet &ts = c->tunes[c->tune_k[q.K]];
@@ -301,6 +310,7 @@ void init_jobstate(JobState &js) {
   js.theta = INF;
   js.far_min = INF;
   js.minc = INF;
+  js.n_alive = -1;
 }
 
 // Grid of the select and grow launches: one wave of resident blocks. Fewer waves means
@@ -319,7 +329,10 @@ inline unsigned grid_for(long long items, long long per_block, unsigned cap) {
 // Launch the init kernels of a job on stream st: level 1 of the arena = INF (the merge of
 // each level L >= 2 writes every state of its block, INF included), masks = 0.
 void launch_job_init(hipStream_t st, const Slot &s, const gst_ctx *c, const JobConst &jc) {
-  const long long words = std::min<long long>((long long)s.arena_words, jc.off[2]);
+  // level 1 only (every merge writes all the states of its level), or levels 1..k when
+  // the merges are sparse (a vertex they skip keeps INF)
+  const long long words = jc.sparse ? (long long)c->n * (jc.W - 1)
+                                    : std::min<long long>((long long)s.arena_words, jc.off[2]);
   k_fill_i32<<<grid_for(words / 4, BLK, 8192), BLK, 0, st>>>(s.T, words, INF);
   k_fill_u64<<<grid_for((long long)s.nact * c->n, BLK, 4096), BLK, 0, st>>>(s.act, (long long)s.nact * c->n, 0ull);
   k_fill_u64<<<grid_for(NSTRIPE * NSTX, BLK, 64), BLK, 0, st>>>(s.stx, (long long)NSTRIPE * NSTX, 0ull);
@@ -873,6 +886,7 @@ static int solve_on_slot(gst_ctx *c, Slot &s, const Query &q, int32_t *ebuf, int
       }
       case PH_TABLES:
         k_tables<<<1, BLK, 0, c->st>>>(c->d_jc, c->d_js, 1, g);
+        if (jc.sparse) k_alive<<<grid_for(c->n, BLK, GROW_GRID_CAP), BLK, 0, c->st>>>(c->d_jc, c->d_js, pl, g);
         break;
       case PH_MERGE: {
         const bool bf = hop_bound_first(jc, h.level);

@@ -94,6 +94,9 @@ struct JobConst {
   int32_t tvb;                // vertices per bound-first hop merge tile (a multiple of tv)
   int32_t delta;              // near-far step of the grow frontier (0 = off)
   int32_t passive;            // cohort member: the grow phases run in the cohort kernels
+  int32_t sparse;             // merges of levels >= 2 cover only the vertices of alv (below)
+  int32_t *alv;               // [n] vertices that can lie on an optimal tree (k_alive): the
+                              // arena's unused tail (levels 1..k take n * (W - 1) words)
   // grow tuning of this job (chosen per k): vertices of degree >= split are grown as
   // ceil(deg / chunk) wavefront items (nchunk[v]; 0 = thread item), the others by threads;
   // with vertices numbered by descending degree those are the ids below n_heavy
@@ -126,6 +129,7 @@ struct alignas(16) JobState {
   int32_t minc;               // min cost activated by the last merge (level start)
   int32_t sat;                // a state came within reach of INF (costs may have saturated)
   int32_t lb;                 // lower bound on the optimum (set after level 1; 0 before)
+  int32_t n_alive;            // entries of alv (-1: not built, the merges cover every vertex)
   unsigned long long st[8];
   unsigned long long stl[8][2];  // per level: edges scanned, state-edge relaxations
 };
@@ -137,8 +141,9 @@ constexpr int FAR_ACT = 2;
 
 // Work items of one round, flattened over jobs: off[kind][j] = first item of job j.
 // WK_MERGEB: the bound-first tiles of hop merges (k_merge_hop<true>); WK_MERGE: the others.
+// WK_ALIVE: the vertices k_alive tests after the tables of a job with sparse merges.
 enum WorkKind : int32_t { WK_SEED = 0, WK_THR = 1, WK_WAVE = 2, WK_CLEAR = 3, WK_MERGE = 4, WK_FINAL = 5, WK_FAR = 6,
-                          WK_COLLECT = 7, WK_MERGEB = 8, NWK = 9 };
+                          WK_COLLECT = 7, WK_MERGEB = 8, WK_ALIVE = 9, NWK = 10 };
 struct Plan {
   int32_t njobs, pad;
   long long off[NWK][MAXJOBS + 1];
@@ -410,7 +415,7 @@ __global__ __launch_bounds__(BLK) void k_step(const JobConst *__restrict__ jcs, 
     JobState &s = jss[j];
     step_job(c, s);
     const int ph = s.phase;
-    long long k[NWK] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    long long k[NWK] = {};
     if (ph == PH_SEED) k[WK_SEED] = c.seed_total;
     if (ph == PH_GROW) {
       const int li = c.hop ? s.depth : s.parity;
@@ -419,9 +424,13 @@ __global__ __launch_bounds__(BLK) void k_step(const JobConst *__restrict__ jcs, 
       k[WK_CLEAR] = k[WK_THR] + k[WK_WAVE];
     }
     if (ph == PH_HOPCLEAR) k[WK_CLEAR] = s.cnt[c.D][0] + s.cnt[c.D][1];
+    if (ph == PH_TABLES && c.sparse && !c.passive) k[WK_ALIVE] = n;
     if (ph == PH_MERGE) {
       if (hop_bound_first(c, s.level)) k[WK_MERGEB] = ((long long)n + c.tvb - 1) / c.tvb;
-      else k[WK_MERGE] = ((long long)n + c.tv - 1) / c.tv;
+      else {
+        const long long nv = (c.sparse && !c.passive && s.n_alive >= 0) ? s.n_alive : n;
+        k[WK_MERGE] = (nv + c.tv - 1) / c.tv;
+      }
     }
     // blocks of 64 vertices per list the merge activates into (heights 0..D-1, or one list)
     if (ph == PH_MERGE && s.level < c.K) k[WK_COLLECT] = (((long long)n + 63) / 64) * (c.hop ? c.D : 1);
@@ -1436,6 +1445,7 @@ __global__ __launch_bounds__(BLK) void k_tables(const JobConst *__restrict__ jcs
   const int K = c.K, W = c.W;
   if (threadIdx.x == 0) {
     const int ub = load_ub(s);
+    if (c.sparse && !c.passive) s->n_alive = 0;  // k_alive fills alv after this kernel
     // level-1 distances are exact up to grow_cap(UB, 1) + 1
     long long r = c.hop ? (long long)(ub >> 1) + 1 : (long long)grow_cap(ub, 1) + 1;
     r1s = (int)(r < LB_CAP ? r : LB_CAP);
@@ -1528,8 +1538,9 @@ __host__ __device__ constexpr int merge_row_stride(int W) { return W + 1; }
 // (MAXK-1) * SU) before the first LDS store: one memory latency per tile. lsub is the
 // job's subset table, copied to LDS.
 constexpr int SU = 8;
+// vid (LDS, non-null for sparse merges): the tile's vertex ids, instead of v0 + vi (H = 1).
 __device__ __forceinline__ void stage_levels(const JobConst &c, int L, long long v0, int tv, int H,
-                                             const uint8_t (*lsub)[64], int32_t *rows) {
+                                             const uint8_t (*lsub)[64], int32_t *rows, const int32_t *vid) {
   const int WS = merge_row_stride(c.W);
   const int t = (int)threadIdx.x;
   int32_t x[MAXK - 1][SU];
@@ -1537,11 +1548,22 @@ __device__ __forceinline__ void stage_levels(const JobConst &c, int L, long long
   for (int l = 1; l < MAXK; l++) {
     if (l < L) {
       const int ns = c.nsub[l], n = tv * H * ns;
-      const int32_t *src = c.T + c.off[l] + v0 * H * ns;
+      if (vid) {  // rows of scattered vertices: word i is rank i % ns of vertex vid[i / ns]
+        const UDiv dn(ns);
+        const int32_t *src = c.T + c.off[l];
 #pragma unroll
-      for (int u = 0; u < SU; u++) {
-        const int i = u * BLK + t;
-        x[l - 1][u] = i < n ? src[i] : INF;
+        for (int u = 0; u < SU; u++) {
+          const int i = u * BLK + t;
+          const int r = (int)dn.div((uint32_t)i);
+          x[l - 1][u] = i < n ? src[(size_t)vid[r < tv ? r : 0] * ns + (i - r * ns)] : INF;
+        }
+      } else {
+        const int32_t *src = c.T + c.off[l] + v0 * H * ns;
+#pragma unroll
+        for (int u = 0; u < SU; u++) {
+          const int i = u * BLK + t;
+          x[l - 1][u] = i < n ? src[i] : INF;
+        }
       }
     }
   }
@@ -1584,6 +1606,100 @@ __device__ __forceinline__ void flush_merge(const JobConst *jcs, JobState *jss, 
   minc = INF;
 }
 
+// Vertices that can lie on an optimal tree, after the tables of a non-hop job with
+// sparse merges (the same round, after k_tables): an optimal tree through v costs at
+// least lb_tail(v, every group), the tour and one-label bounds over v's group distances
+// (a closed walk from v around the tree visits every group), so v is listed in alv when
+// that bound is within UB. The distances are exact up to UB (level 1 grows to UB - 1),
+// which makes the bound strong: on C4 at k = 5 it leaves 13-35% of the vertices.
+__global__ __launch_bounds__(BLK) void k_alive(const JobConst *__restrict__ jcs, JobState *jss, const Plan *pl, GraphDev g) {
+  __shared__ long long soff[MAXJOBS + 1];
+  // per wavefront: the bound's tables for X = every group (the same for all vertices of a
+  // job): hk[a][b][R] at [a * MAXK + b], w2[a][R] at [MAXK * MAXK + a]; the UB and r1
+  __shared__ int tabs[BLK / WAVE][MAXK * MAXK + MAXK + 2];
+  const int nj = load_plan(pl, WK_ALIVE, soff);
+  const long long total = soff[nj];
+  const int lane = lane_id(), wv = threadIdx.x >> 6;
+  int *T = tabs[wv];
+  int cached = -1;
+  for (long long wbase = (long long)blockIdx.x * BLK + (threadIdx.x & ~63); wbase < total;
+       wbase += (long long)gridDim.x * BLK) {
+    const long long it = wbase + lane;
+    const int jf = __builtin_amdgcn_readfirstlane(find_job(soff, nj, wbase));
+    const long long wl = wbase + WAVE - 1 < total ? wbase + WAVE - 1 : total - 1;
+    const bool uni = jf == __builtin_amdgcn_readfirstlane(find_job(soff, nj, wl));
+    int jb = jf;
+    if (!uni && it < total) jb = find_job(soff, nj, it);
+    // (re)load the tables of jf; a lane of a wave that spans two jobs reads its own job's
+    // tables from global memory instead
+    if (jf != cached) {
+      __builtin_amdgcn_wave_barrier();  // the previous job's tables have been read
+      const JobConst &c = jcs[jf];
+      const int K = c.K, W = c.W, R = c.R;
+      if (lane < MAXK * MAXK) {
+        const int a = lane / MAXK, b = lane % MAXK;
+        T[lane] = (a < K && b < K) ? c.tab[(a * K + b) * W + R] : 0;
+      } else if (lane < MAXK * MAXK + MAXK) {
+        const int a = lane - MAXK * MAXK;
+        T[lane] = a < K ? c.tab[K * K * W + a * W + R] : 0;
+      } else if (lane == MAXK * MAXK + MAXK) {
+        T[lane] = load_ub(&jss[jf]);
+      } else if (lane == MAXK * MAXK + MAXK + 1) {
+        T[lane] = jss[jf].r1;
+      }
+      __builtin_amdgcn_wave_barrier();
+      cached = jf;
+    }
+    bool alive = false;
+    int32_t v = 0;
+    if (it < total) {
+      const JobConst &c = jcs[jb];
+      const int K = c.K;
+      v = (int32_t)(it - soff[jb]);
+      const int32_t *sing = c.T + c.off[1] + (size_t)v * K;  // level-1 row (H = 1)
+      if (jb == jf) {
+        // lb_tail(v, every group), with the tables from LDS
+        const int r1 = T[MAXK * MAXK + MAXK + 1];
+        int dl[MAXK];
+        int one = 0, mind = INF;
+#pragma unroll
+        for (int i = 0; i < MAXK; i++) {
+          int x = i < K ? sing[i] : 0;
+          x = x < r1 ? x : r1;
+          x = x < LB_CAP ? x : LB_CAP;
+          dl[i] = x;
+          if (i < K) { one = x > one ? x : one; mind = x < mind ? x : mind; }
+        }
+        int lb1 = INF, lb2 = 0;
+#pragma unroll
+        for (int a = 0; a < MAXK; a++) {
+          if (a >= K) continue;
+#pragma unroll
+          for (int b = 0; b < MAXK; b++) {
+            if (b >= K) continue;
+            const int t = (dl[a] + T[a * MAXK + b] + dl[b]) >> 1;
+            lb1 = t < lb1 ? t : lb1;
+          }
+          const int t2 = (dl[a] + T[MAXK * MAXK + a] + mind) >> 1;
+          lb2 = t2 > lb2 ? t2 : lb2;
+        }
+        int lb = lb1 > lb2 ? lb1 : lb2;
+        lb = lb > one ? lb : one;
+        alive = lb <= T[MAXK * MAXK + MAXK];
+      } else {
+        JobState *s = &jss[jb];
+        alive = lb_tail(sing, s->r1, 0, c, c.tab, c.tab + c.K * c.K * c.W) <= load_ub(s);
+      }
+    }
+    if (uni) {  // one append per wavefront
+      const int pos = wave_append(&jss[jf].n_alive, alive ? 1 : 0);
+      if (alive) jcs[jf].alv[pos] = v;
+    } else if (alive) {
+      jcs[jb].alv[atomicAdd(&jss[jb].n_alive, 1)] = v;
+    }
+  }
+}
+
 // Non-hop merge for level L: T(v,p) = min over splits p = p1 + p2 of T(v,p1) + T(v,p2).
 // A tile of vertices has its rows staged in LDS (coalesced contiguous read), each
 // thread evaluates (vertex, subset) items, new states are activated for growth.
@@ -1591,12 +1707,13 @@ __device__ __forceinline__ void flush_merge(const JobConst *jcs, JobState *jss, 
 // block may take 32 KB of the CU's 160 KB (5888 words: 45 rows of 129 words at k = 7, 178
 // of 33 at k = 5) without losing a block
 constexpr int MERGE_LDS_WORDS = 5888;
-__global__ __launch_bounds__(BLK) void k_merge(const JobConst *__restrict__ jcs, JobState *jss, const Plan *pl, GraphDev g) {
+__global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(5))) void k_merge(const JobConst *__restrict__ jcs, JobState *jss, const Plan *pl, GraphDev g) {
   __shared__ long long soff[MAXJOBS + 1];
   __shared__ int32_t rows[MERGE_LDS_WORDS];
   __shared__ unsigned long long amask[512];
   __shared__ int ubs, minc_s;
   __shared__ uint32_t lsubw[128];
+  __shared__ int32_t tvid[512];  // sparse merges: the tile's vertices (entries of alv)
   uint8_t (*lsub)[64] = (uint8_t (*)[64])lsubw;
   const int nj = load_plan(pl, WK_MERGE, soff);
   const long long total = soff[nj];
@@ -1618,14 +1735,23 @@ __global__ __launch_bounds__(BLK) void k_merge(const JobConst *__restrict__ jcs,
   const int r1 = s->r1;
   int merged = 0;
   {
+    // Sparse merges (after k_alive): the tile is TV consecutive entries of alv, the
+    // vertices whose full-tree lower bound is within UB; every other vertex lies on no
+    // optimal tree and keeps the INF its level was filled with at query start.
+    const bool sp = c.sparse && !c.passive && s->n_alive >= 0;
+    const long long nv = sp ? s->n_alive : g.n;
     const long long v0 = (gt - soff[jb]) * TV;
-    const int tv = (int)min((long long)TV, (long long)g.n - v0);
+    const int tv = (int)min((long long)TV, nv - v0);
+    if (sp) {
+      for (int i = threadIdx.x; i < tv; i += BLK) tvid[i] = c.alv[v0 + i];
+      __syncthreads();
+    }
     // stage the lower levels of the tile's rows into LDS, indexed by subset p
     // every subset read below has fewer than L members, so staging overwrites all of it
     for (int i = threadIdx.x; i < tv; i += BLK) amask[i] = 0;
     if (threadIdx.x == 0) { ubs = load_ub(s); minc_s = INF; }
     load_lsub(c, lsub);
-    stage_levels(c, L, v0, tv, 1, lsub, rows);
+    stage_levels(c, L, v0, tv, 1, lsub, rows, sp ? tvid : nullptr);
     __syncthreads();
     const int ub = ubs;
     int ubl = ub, mact = INF;
@@ -1647,10 +1773,11 @@ __global__ __launch_bounds__(BLK) void k_merge(const JobConst *__restrict__ jcs,
       }
       // every state of the level is written here, INF included: levels >= 2 are not
       // filled at query start
+      const long long v = sp ? (long long)tvid[vi] : v0 + vi;
       if (c.passive && L < K)
-        as_global(c.cT)[(size_t)(v0 + vi) * cohort_stride(c.cB, nL) + c.cq * nL + j] = m < INF ? m : INF;
+        as_global(c.cT)[(size_t)v * cohort_stride(c.cB, nL) + c.cq * nL + j] = m < INF ? m : INF;
       else
-        as_global(c.T)[(size_t)c.off[L] + (size_t)(v0 + vi) * nL + j] = m < INF ? m : INF;
+        as_global(c.T)[(size_t)c.off[L] + (size_t)v * nL + j] = m < INF ? m : INF;
       if (m >= INF) continue;
       if (m >= SAT_AT) flag_saturation(s);
       merged++;
@@ -1686,7 +1813,7 @@ __global__ __launch_bounds__(BLK) void k_merge(const JobConst *__restrict__ jcs,
       }
     } else {
       for (int vi = threadIdx.x; vi < tv; vi += BLK)  // k_collect builds the worklist
-        if (amask[vi]) c.act[(size_t)lo * g.n + v0 + vi] = amask[vi];
+        if (amask[vi]) c.act[(size_t)lo * g.n + (sp ? (long long)tvid[vi] : v0 + vi)] = amask[vi];
     }
     __syncthreads();
   }
