@@ -63,6 +63,7 @@ struct Slot {
   unsigned long long *act = nullptr;
   int32_t *lst = nullptr, *blst = nullptr, *tab = nullptr, *stk = nullptr, *out = nullptr;
   unsigned long long *stx = nullptr;  // striped work counters
+  unsigned long long *alb = nullptr;  // non-hop: the alive bitmap of k_alive, a bit per vertex
   size_t bytes = 0;
   int nact = 0, nlst = 0, nblst = 0;  // mask arrays, thread lists, wavefront lists
   int ecap = 0;                       // tree edges the out buffer holds
@@ -162,6 +163,7 @@ size_t slot_carve(const gst_ctx *c, Slot &s, char *base, int K, int D, int ecap)
   s.stk = (int32_t *)take((size_t)SCAP * 12);
   s.out = (int32_t *)take((size_t)ecap * 12);
   s.stx = (unsigned long long *)take((size_t)NSTRIPE * NSTX * 8);
+  s.alb = D < 0 ? (unsigned long long *)take((size_t)(c->n + 63) / 64 * 8) : nullptr;
   s.bytes = off;
   return off;
 }
@@ -257,6 +259,7 @@ void fill_jobconst(const gst_ctx *c, const Slot &s, const Query &q, JobConst &jc
   // the arena's unused tail (levels 1..k take n * (W - 1) of its n * W words)
   jc.sparse = GST_SPARSE_MERGE && q.D < 0 && !(q.flags & GST_F_NO_PRUNE);
   jc.alv = s.T + (size_t)c->n * (jc.W - 1);
+  jc.alb = s.alb;
   jc.delta = q.D >= 0 ? 0 : c->delta;
   {
     const gst_ctx::TuneSet &ts = c->tunes[c->tune_k[q.K]];

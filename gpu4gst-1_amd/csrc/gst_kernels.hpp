@@ -97,6 +97,8 @@ struct JobConst {
   int32_t sparse;             // merges of levels >= 2 cover only the vertices of alv (below)
   int32_t *alv;               // [n] vertices that can lie on an optimal tree (k_alive): the
                               // arena's unused tail (levels 1..k take n * (W - 1) words)
+  unsigned long long *alb;    // [ceil(n / 64)] bit v: v is listed in alv; grows of levels >= 2
+                              // skip targets whose bit is clear
   // grow tuning of this job (chosen per k): vertices of degree >= split are grown as
   // ceil(deg / chunk) wavefront items (nchunk[v]; 0 = thread item), the others by threads;
   // with vertices numbered by descending degree those are the ids below n_heavy
@@ -424,7 +426,9 @@ __global__ __launch_bounds__(BLK) void k_step(const JobConst *__restrict__ jcs, 
       k[WK_CLEAR] = k[WK_THR] + k[WK_WAVE];
     }
     if (ph == PH_HOPCLEAR) k[WK_CLEAR] = s.cnt[c.D][0] + s.cnt[c.D][1];
-    if (ph == PH_TABLES && c.sparse && !c.passive) k[WK_ALIVE] = n;
+    // padded to whole wavefronts: every wavefront of k_alive then tests 64 vertices of one
+    // job, which are one word of its alive bitmap
+    if (ph == PH_TABLES && c.sparse && !c.passive) k[WK_ALIVE] = ((long long)n + WAVE - 1) & ~(long long)(WAVE - 1);
     if (ph == PH_MERGE) {
       if (hop_bound_first(c, s.level)) k[WK_MERGEB] = ((long long)n + c.tvb - 1) / c.tvb;
       else {
@@ -1029,17 +1033,31 @@ template <int NE>
 __device__ __forceinline__ void relax_edges(int32_t *dbase, size_t dstride, unsigned long long *act_lo,
                                             const int32_t *nchk, int32_t nheavy, const RelaxGroup &G, const bool (&h)[NE],
                                             const int32_t (&u)[NE], const int32_t (&w)[NE], bool (&want)[NE],
-                                            int (&nch)[NE], int &n_atom, int ub, int acap, int &n_rel) {
+                                            int (&nch)[NE], int &n_atom, int ub, int acap, int &n_rel,
+                                            const unsigned long long *alb) {
   gi32 *Td[NE];
   gu64 *const act = as_global(act_lo);
   int t[NE][RG];  // 0 where no relaxation is made: no candidate (>= 1) is below it
+  int lj[NE];     // the largest source cost arc j relaxes (-1: none)
+#pragma unroll
+  for (int j = 0; j < NE; j++) lj[j] = h[j] ? ub - w[j] : -1;
+  if (alb) {  // levels >= 2 of a job with alive lists: no state of a dead target is made
+    int cmin = INF;  // the group's cheapest state: arcs with lj below it relax nothing
+#pragma unroll
+    for (int i = 0; i < RG; i++) cmin = (i < G.n && G.c[i] < cmin) ? G.c[i] : cmin;
+    const gi32 *ab32 = as_global((const int32_t *)alb);  // little-endian halves of the words
+    uint32_t ab[NE];
+#pragma unroll
+    for (int j = 0; j < NE; j++) ab[j] = lj[j] >= cmin ? (uint32_t)ab32[u[j] >> 5] : 0u;
+#pragma unroll
+    for (int j = 0; j < NE; j++) lj[j] = ((ab[j] >> (u[j] & 31)) & 1) ? lj[j] : -1;
+  }
 #pragma unroll
   for (int j = 0; j < NE; j++) {
     Td[j] = as_global(dbase + (size_t)u[j] * dstride);
-    const int lj = h[j] ? ub - w[j] : -1;  // the largest source cost this arc relaxes
 #pragma unroll
     for (int i = 0; i < RG; i++) {
-      const bool r = i < G.n && G.c[i] <= lj;
+      const bool r = i < G.n && G.c[i] <= lj[j];
       n_rel += (int)r;
       t[j][i] = r ? Td[j][G.p[i]] : 0;
     }
@@ -1053,7 +1071,7 @@ __device__ __forceinline__ void relax_edges(int32_t *dbase, size_t dstride, unsi
       if (h[j] && i < G.n) {
         int nc = G.c[i] + w[j];
         nc = nc < INF ? nc : INF;
-        if (nc < t[j][i]) {
+        if (nc < t[j][i]) {  // t is 0 where lj excluded the relaxation
           __hip_atomic_fetch_min(&Td[j][G.p[i]], nc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           n_atom++;
           if (nc <= acap) imp[j] |= 1ull << G.p[i];
@@ -1094,6 +1112,18 @@ __device__ __forceinline__ int arcs_within(const int32_t *wts, int32_t e0, int d
 // Upper bound a job's grow kernels cut relaxations at (INF when it does not prune).
 __device__ __forceinline__ int relax_ub(const JobConst &c, const JobState *s) {
   return (c.flags & JF_NO_PRUNE) ? INF : load_ub(s);
+}
+
+// The alive bitmap a job's grows of level L skip dead targets by (nullptr: none): built by
+// k_alive after the tables for non-hop pruned jobs. A vertex whose full-tree lower bound
+// exceeds UB lies on no optimal tree, so no state of it at level >= 2 belongs to an
+// optimal derivation; its merged states are never written (they keep the INF of the
+// query-start fill), and with this its grown ones are not either.
+#ifndef GST_ALIVE_GROW
+#define GST_ALIVE_GROW 1
+#endif
+__device__ __forceinline__ const unsigned long long *grow_alb(const JobConst &c, const JobState *s, int L) {
+  return (GST_ALIVE_GROW && c.sparse && !c.passive && L >= 2 && s->n_alive >= 0) ? c.alb : nullptr;
 }
 
 // Largest cost at which a grown state is activated: non-hop states above floor(UB/2) are
@@ -1159,6 +1189,7 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) void k
   int32_t *udst = nullptr;
   const unsigned long long *uacur = nullptr;
   unsigned long long *uanxt = nullptr;
+  const unsigned long long *ualb = nullptr;  // the job's alive bitmap when its grows use it
   for (long long wbase = (long long)blockIdx.x * BLK + (threadIdx.x & ~63); wbase < total; wbase += stride) {
     const long long it = wbase + lane;
     int deg = 0, iub = INF;
@@ -1184,6 +1215,7 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) void k
       udst = lrow(c, L, 0, c.hop ? d + 1 : 0);
       unck = c.nchunk;
       uheavy = c.n_heavy;
+      ualb = grow_alb(c, s, L);
     }
     if (it < total) {
       int jb, lo;
@@ -1269,7 +1301,7 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) void k
 #pragma unroll
           for (int i = 0; i < RG; i++) { Gj.p[i] = S.gp[j][i]; Gj.c[i] = S.gc[j][i]; }
           relax_edges<1>(S.dbase[j], (size_t)S.dstride[j], S.act[j], S.nck[j], S.nhv[j], Gj, h, u, w, want, nch, k.atom,
-                         S.ub[j], S.acap[j], k.relax);
+                         S.ub[j], S.acap[j], k.relax, uni ? ualb : nullptr);
           ljb = S.jb[j];
           llo = S.lo[j];
         }
@@ -1310,6 +1342,7 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) void k
   int32_t *jsrc = nullptr, *jdst = nullptr;
   const int32_t *jnck = nullptr;
   unsigned long long *jact = nullptr;
+  const unsigned long long *jalb = nullptr;
   // The wave's items are it0, it0 + S, it0 + 2S, ... (S = waves in the grid). Each lane
   // fetches the header of one of the next 64 (vertex, chunk, active mask, arc range, job),
   // so the header loads of 64 items share one memory latency; the wave then walks them.
@@ -1353,6 +1386,7 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) void k
       // value read earlier in the launch still bounds every optimal derivation
       jub = __builtin_amdgcn_readfirstlane(relax_ub(c, s));
       jacap = act_cap(c, jub, jL);
+      jalb = grow_alb(c, s, jL);
     }
     const int lo = jlo;
     const size_t dstride = (size_t)jrow;
@@ -1389,7 +1423,7 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) void k
         }
         if (__builtin_amdgcn_readfirstlane(w[0]) > lim) break;  // lane 0 holds arc e
         if (lane == 0) k.edges += min(GW_NE * WAVE, ee - e);
-        relax_edges<GW_NE>(dbase, dstride, act_lo, jnck, jheavy, G, h, u, w, want, nch, k.atom, ub, jacap, k.relax);
+        relax_edges<GW_NE>(dbase, dstride, act_lo, jnck, jheavy, G, h, u, w, want, nch, k.atom, ub, jacap, k.relax, jalb);
 #pragma unroll
         for (int j = 0; j < GW_NE; j++) {
           wp_push(pb, jcs, jss, g, jb, lo, want[j], u[j], nch[j]);
@@ -1606,36 +1640,53 @@ __device__ __forceinline__ void flush_merge(const JobConst *jcs, JobState *jss, 
   minc = INF;
 }
 
+// Appends the nb buffered alive vertices of job bj (-1: none) to its list. Called
+// converged by the whole wavefront.
+constexpr int ALIVE_BUF = 512;
+__device__ __forceinline__ void alive_flush(const JobConst *jcs, JobState *jss, int bj, const int32_t *buf, int nb) {
+  __builtin_amdgcn_wave_barrier();
+  if (bj >= 0 && nb) {
+    const int lane = lane_id();
+    int base = 0;
+    if (lane == 0) base = atomicAdd(&jss[bj].n_alive, nb);
+    base = __shfl(base, 0, 64);
+    int32_t *dst = jcs[bj].alv + base;
+    for (int i = lane; i < nb; i += WAVE) dst[i] = buf[i];
+  }
+  __builtin_amdgcn_wave_barrier();
+}
+
 // Vertices that can lie on an optimal tree, after the tables of a non-hop job with
 // sparse merges (the same round, after k_tables): an optimal tree through v costs at
 // least lb_tail(v, every group), the tour and one-label bounds over v's group distances
-// (a closed walk from v around the tree visits every group), so v is listed in alv when
-// that bound is within UB. The distances are exact up to UB (level 1 grows to UB - 1),
-// which makes the bound strong: on C4 at k = 5 it leaves 13-35% of the vertices.
+// (a closed walk from v around the tree visits every group), so v is alive when that
+// bound is within UB. The distances are exact up to UB (level 1 grows to UB - 1), which
+// makes the bound strong: on C4 at k = 5 it leaves 13-35% of the vertices. Alive vertices
+// are appended to alv, and every vertex's bit of alb is written (set when alive).
+// The plan pads each job to a multiple of 64 vertices, so a wavefront (64 consecutive
+// items from a multiple of 64) tests 64 vertices of one job, one word of its alb.
 __global__ __launch_bounds__(BLK) void k_alive(const JobConst *__restrict__ jcs, JobState *jss, const Plan *pl, GraphDev g) {
   __shared__ long long soff[MAXJOBS + 1];
   // per wavefront: the bound's tables for X = every group (the same for all vertices of a
   // job): hk[a][b][R] at [a * MAXK + b], w2[a][R] at [MAXK * MAXK + a]; the UB and r1
   __shared__ int tabs[BLK / WAVE][MAXK * MAXK + MAXK + 2];
+  // per wavefront: alive vertices of job bj not yet appended to its list. One job counter
+  // takes every append of the grid, so appends are batched to one atomicAdd per flush.
+  __shared__ int32_t abuf[BLK / WAVE][ALIVE_BUF];
   const int nj = load_plan(pl, WK_ALIVE, soff);
   const long long total = soff[nj];
   const int lane = lane_id(), wv = threadIdx.x >> 6;
   int *T = tabs[wv];
-  int cached = -1;
+  int32_t *buf = abuf[wv];
+  int cached = -1, bj = -1, nb = 0;  // wave-uniform
   for (long long wbase = (long long)blockIdx.x * BLK + (threadIdx.x & ~63); wbase < total;
        wbase += (long long)gridDim.x * BLK) {
-    const long long it = wbase + lane;
     const int jf = __builtin_amdgcn_readfirstlane(find_job(soff, nj, wbase));
-    const long long wl = wbase + WAVE - 1 < total ? wbase + WAVE - 1 : total - 1;
-    const bool uni = jf == __builtin_amdgcn_readfirstlane(find_job(soff, nj, wl));
-    int jb = jf;
-    if (!uni && it < total) jb = find_job(soff, nj, it);
-    // (re)load the tables of jf; a lane of a wave that spans two jobs reads its own job's
-    // tables from global memory instead
-    if (jf != cached) {
+    const JobConst &c = jcs[jf];
+    const int K = c.K;
+    if (jf != cached) {  // (re)load the tables of jf
       __builtin_amdgcn_wave_barrier();  // the previous job's tables have been read
-      const JobConst &c = jcs[jf];
-      const int K = c.K, W = c.W, R = c.R;
+      const int W = c.W, R = c.R;
       if (lane < MAXK * MAXK) {
         const int a = lane / MAXK, b = lane % MAXK;
         T[lane] = (a < K && b < K) ? c.tab[(a * K + b) * W + R] : 0;
@@ -1650,54 +1701,51 @@ __global__ __launch_bounds__(BLK) void k_alive(const JobConst *__restrict__ jcs,
       __builtin_amdgcn_wave_barrier();
       cached = jf;
     }
+    const int32_t v0 = (int32_t)(wbase - soff[jf]);  // a multiple of 64
+    const int32_t v = v0 + lane;
     bool alive = false;
-    int32_t v = 0;
-    if (it < total) {
-      const JobConst &c = jcs[jb];
-      const int K = c.K;
-      v = (int32_t)(it - soff[jb]);
+    if (v < g.n) {
+      // lb_tail(v, every group), with the tables from LDS
       const int32_t *sing = c.T + c.off[1] + (size_t)v * K;  // level-1 row (H = 1)
-      if (jb == jf) {
-        // lb_tail(v, every group), with the tables from LDS
-        const int r1 = T[MAXK * MAXK + MAXK + 1];
-        int dl[MAXK];
-        int one = 0, mind = INF;
+      const int r1 = T[MAXK * MAXK + MAXK + 1];
+      int dl[MAXK];
+      int one = 0, mind = INF;
 #pragma unroll
-        for (int i = 0; i < MAXK; i++) {
-          int x = i < K ? sing[i] : 0;
-          x = x < r1 ? x : r1;
-          x = x < LB_CAP ? x : LB_CAP;
-          dl[i] = x;
-          if (i < K) { one = x > one ? x : one; mind = x < mind ? x : mind; }
-        }
-        int lb1 = INF, lb2 = 0;
-#pragma unroll
-        for (int a = 0; a < MAXK; a++) {
-          if (a >= K) continue;
-#pragma unroll
-          for (int b = 0; b < MAXK; b++) {
-            if (b >= K) continue;
-            const int t = (dl[a] + T[a * MAXK + b] + dl[b]) >> 1;
-            lb1 = t < lb1 ? t : lb1;
-          }
-          const int t2 = (dl[a] + T[MAXK * MAXK + a] + mind) >> 1;
-          lb2 = t2 > lb2 ? t2 : lb2;
-        }
-        int lb = lb1 > lb2 ? lb1 : lb2;
-        lb = lb > one ? lb : one;
-        alive = lb <= T[MAXK * MAXK + MAXK];
-      } else {
-        JobState *s = &jss[jb];
-        alive = lb_tail(sing, s->r1, 0, c, c.tab, c.tab + c.K * c.K * c.W) <= load_ub(s);
+      for (int i = 0; i < MAXK; i++) {
+        int x = i < K ? sing[i] : 0;
+        x = x < r1 ? x : r1;
+        x = x < LB_CAP ? x : LB_CAP;
+        dl[i] = x;
+        if (i < K) { one = x > one ? x : one; mind = x < mind ? x : mind; }
       }
+      int lb1 = INF, lb2 = 0;
+#pragma unroll
+      for (int a = 0; a < MAXK; a++) {
+        if (a >= K) continue;
+#pragma unroll
+        for (int b = 0; b < MAXK; b++) {
+          if (b >= K) continue;
+          const int t = (dl[a] + T[a * MAXK + b] + dl[b]) >> 1;
+          lb1 = t < lb1 ? t : lb1;
+        }
+        const int t2 = (dl[a] + T[MAXK * MAXK + a] + mind) >> 1;
+        lb2 = t2 > lb2 ? t2 : lb2;
+      }
+      int lb = lb1 > lb2 ? lb1 : lb2;
+      lb = lb > one ? lb : one;
+      alive = lb <= T[MAXK * MAXK + MAXK];
     }
-    if (uni) {  // one append per wavefront
-      const int pos = wave_append(&jss[jf].n_alive, alive ? 1 : 0);
-      if (alive) jcs[jf].alv[pos] = v;
-    } else if (alive) {
-      jcs[jb].alv[atomicAdd(&jss[jb].n_alive, 1)] = v;
+    const unsigned long long m = __ballot(alive);
+    if (v0 < g.n && lane == 0) c.alb[v0 >> 6] = m;
+    if (bj != jf || nb + WAVE > ALIVE_BUF) {
+      alive_flush(jcs, jss, bj, buf, nb);
+      bj = jf;
+      nb = 0;
     }
+    if (alive) buf[nb + __popcll(m & ((1ull << lane) - 1))] = v;
+    nb += __popcll(m);
   }
+  alive_flush(jcs, jss, bj, buf, nb);
 }
 
 // Non-hop merge for level L: T(v,p) = min over splits p = p1 + p2 of T(v,p1) + T(v,p2).
