@@ -167,10 +167,9 @@ __global__ __launch_bounds__(BLK) void k_cgrow(const CohortConst *__restrict__ c
   for (long long it = (long long)blockIdx.x * (BLK / WAVE) + wave; it < total; it += waves) {
     const int32_t v = __builtin_amdgcn_readfirstlane(itm[2 * it]);
     const int32_t ch = __builtin_amdgcn_readfirstlane(itm[2 * it + 1]);
-    unsigned long long m[NWT];
-    int32_t sv[NWT], cap[NWT];  // cap: the member's floor(UB/2), the largest value grown
+    int32_t sv[NWT], cap[NWT];  // cap: the member's growth cap, the largest value activated
     int32_t lim[NWT];           // the heaviest arc a grown label relaxes: UB - its value
-    int nb = 0, lmax = -1;
+    int nb = 0;
 #pragma unroll
     for (int c = 0; c < NWT; c++) {
       unsigned long long x = c < NW ? actc[(size_t)v * NW + c] : 0ull;
@@ -189,10 +188,7 @@ __global__ __launch_bounds__(BLK) void k_cgrow(const CohortConst *__restrict__ c
       }
       const bool grow = bit && sv[c] <= cap[c];
       lim[c] = grow ? ubq - sv[c] : -1;
-      lmax = lim[c] > lmax ? lim[c] : lmax;
-      x = __ballot(grow);
-      m[c] = x;
-      nb += __popcll(x);
+      nb += __popcll(__ballot(grow));
     }
     if (!nb) continue;
     // rows are sorted by weight: the scan ends at the first arc heavier than every lim,
@@ -207,12 +203,13 @@ __global__ __launch_bounds__(BLK) void k_cgrow(const CohortConst *__restrict__ c
       }
       cmx[c] = __builtin_amdgcn_readfirstlane(x);
     }
-    lmax = -1;
+    int lmax = -1;
 #pragma unroll
     for (int c = 0; c < NWT; c++) lmax = cmx[c] > lmax ? cmx[c] : lmax;
     const int32_t vb = g.beg[v], ve = g.beg[v + 1];
     const int32_t eb = ch < 0 ? vb : vb + ch * cc.chunk;
     const int32_t ee = ch < 0 ? ve : min(ve, eb + cc.chunk);
+    uint32_t nr = 0;  // relaxations of this lane in this item
     for (int32_t e0 = eb; e0 < ee; e0 += WAVE) {
       const int na = min(WAVE, ee - e0);
       const int32_t au = lane < na ? adj[e0 + lane] : 0;
@@ -220,46 +217,56 @@ __global__ __launch_bounds__(BLK) void k_cgrow(const CohortConst *__restrict__ c
       if (__builtin_amdgcn_readfirstlane(aw) > lmax) break;  // lane 0 holds arc e0
       n_edges += na;
       for (int j = 0; j < na; j += AU) {  // uniform trip count
+        // wt: the arc's weight, or 2^31 - 1 past the strip's arcs, so that one uniform
+        // test against a chunk's lim maximum also covers the strip's end
         int32_t u[AU], wt[AU];
-        bool ok[AU];
 #pragma unroll
         for (int a = 0; a < AU; a++) {
           u[a] = __builtin_amdgcn_readlane(au, min(j + a, WAVE - 1));
-          wt[a] = __builtin_amdgcn_readlane(aw, min(j + a, WAVE - 1));
-          ok[a] = j + a < na && wt[a] <= lmax;
+          wt[a] = j + a < na ? __builtin_amdgcn_readlane(aw, min(j + a, WAVE - 1)) : 0x7fffffff;
         }
-        int32_t t[AU][NWT];  // 0 where no relaxation is made: no candidate (>= 1) is below it
+        // t: the target label, 0 where no relaxation is made (no candidate, >= 1, is
+        // below it); lanes of labels not grown have lim = -1 and make none
+        int32_t t[AU][NWT];
 #pragma unroll
         for (int a = 0; a < AU; a++)
 #pragma unroll
           for (int c = 0; c < NWT; c++) {
             t[a][c] = 0;
-            if (ok[a] && wt[a] <= cmx[c]) {  // uniform
-              const bool r = wt[a] <= lim[c];  // lim is -1 for labels not grown
-              n_relax += (unsigned long long)r;
+            if (wt[a] <= cmx[c]) {  // uniform
+              const bool r = wt[a] <= lim[c];
+              nr += (uint32_t)r;
               if (r) t[a][c] = CT[(size_t)u[a] * S + c * 64 + lane];
             }
           }
 #pragma unroll
         for (int a = 0; a < AU; a++) {
-          if (!ok[a]) continue;
-          unsigned long long imc = 0;  // lane c: improved bits of chunk c
+          if (wt[a] > lmax) continue;  // uniform
+          // lane c: the improved bits of chunk c to activate
+          int32_t imlo = 0, imhi = 0;
           bool any = false;
 #pragma unroll
           for (int c = 0; c < NWT; c++) {
             if (wt[a] > cmx[c]) continue;  // uniform (also every chunk without grown labels)
             int nc = sv[c] + wt[a];
             nc = nc < INF ? nc : INF;
-            const bool imp = ((m[c] >> lane) & 1) && nc < t[a][c];
+            const bool imp = nc < t[a][c];
+            // activated only where the select pass would grow it (<= the member's cap)
+            const bool act = imp && nc <= cap[c];
             if (imp)
               __hip_atomic_fetch_min(&CT[(size_t)u[a] * S + c * 64 + lane], nc, __ATOMIC_RELAXED,
                                      __HIP_MEMORY_SCOPE_AGENT);
-            // activated only where the select pass would grow it (<= the member's UB/2)
-            const unsigned long long im = __ballot(imp && nc <= cap[c]);
-            if (lane == c) imc = im;
-            any = any || im != 0;
+            const unsigned long long im = __builtin_amdgcn_ballot_w64(act);
+            if (im) {  // uniform
+              if (lane == c) {
+                imlo = (int32_t)(uint32_t)im;
+                imhi = (int32_t)(uint32_t)(im >> 32);
+              }
+              any = true;
+            }
           }
           if (any) {
+            const unsigned long long imc = ((unsigned long long)(uint32_t)imhi << 32) | (uint32_t)imlo;
             if (imc)
               __hip_atomic_fetch_or(&actn[(size_t)u[a] * NW + lane], imc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (lane == 0) flagn[u[a]] = 1;
@@ -267,6 +274,7 @@ __global__ __launch_bounds__(BLK) void k_cgrow(const CohortConst *__restrict__ c
         }
       }
     }
+    n_relax += nr;
   }
   for (int off = 32; off > 0; off >>= 1) n_relax += __shfl_xor(n_relax, off, 64);  // per-lane counts
   if (lane == 0 && n_edges) {
