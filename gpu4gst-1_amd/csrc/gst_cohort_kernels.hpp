@@ -147,7 +147,7 @@ __global__ __launch_bounds__(BLK) void k_cselect(const CohortConst *__restrict__
 // label above its member's UB is not made, and since rows are sorted by weight the arc
 // scan of an item ends at the first arc too heavy for every grown label.
 template <int NWT, int AU>
-__global__ __launch_bounds__(BLK) void k_cgrow(const CohortConst *__restrict__ ccp, const CohortState *__restrict__ csp,
+__global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(NWT <= 5 ? 5 : 1))) void k_cgrow(const CohortConst *__restrict__ ccp, const CohortState *__restrict__ csp,
                                                const JobConst *jcs, const JobState *jss, GraphDev g) {
   const CohortConst &cc = *ccp;
   const int cur = csp->par, nx = cur ^ 1;
@@ -163,6 +163,21 @@ __global__ __launch_bounds__(BLK) void k_cgrow(const CohortConst *__restrict__ c
   const gi32 *itm = as_global(cc.items + (size_t)cur * cc.icap * 2);
   const int32_t *__restrict__ adj = g.adj;
   const int32_t *__restrict__ wts = g.w;
+  // per label of a row: its member's UB (INF when the member does not prune), read once
+  // per block instead of through two dependent loads per item. UB only decreases, so the
+  // value at launch bounds every optimal derivation through the launch (at level 1 it
+  // does not change during the rounds at all: k_cstar lowers it after them).
+  __shared__ int32_t lub[NWT * 64];
+  for (int l = threadIdx.x; l < NWT * 64; l += BLK) {
+    int32_t x = INF;
+    const int q = l / cc.C;
+    if (l < S && q < cc.B) {
+      const int jb = cc.jobs[q];
+      if (!(jcs[jb].flags & JF_NO_PRUNE)) x = load_ub(&jss[jb]);
+    }
+    lub[l] = x;
+  }
+  __syncthreads();
   unsigned long long n_edges = 0, n_relax = 0;
   for (long long it = (long long)blockIdx.x * (BLK / WAVE) + wave; it < total; it += waves) {
     const int32_t v = __builtin_amdgcn_readfirstlane(itm[2 * it]);
@@ -177,15 +192,8 @@ __global__ __launch_bounds__(BLK) void k_cgrow(const CohortConst *__restrict__ c
           (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)x);
       const bool bit = (x >> lane) & 1;
       sv[c] = bit ? CT[(size_t)v * S + c * 64 + lane] : INF;
-      cap[c] = INF;
-      int ubq = INF;
-      if (bit) {
-        const int q = (c * 64 + lane) / cc.C;
-        if (!(jcs[cc.jobs[q]].flags & JF_NO_PRUNE)) {
-          ubq = load_ub(&jss[cc.jobs[q]]);
-          cap[c] = grow_cap(ubq, cc.L);
-        }
-      }
+      const int ubq = lub[c * 64 + lane];
+      cap[c] = ubq < INF ? grow_cap(ubq, cc.L) : INF;
       const bool grow = bit && sv[c] <= cap[c];
       lim[c] = grow ? ubq - sv[c] : -1;
       nb += __popcll(__ballot(grow));
