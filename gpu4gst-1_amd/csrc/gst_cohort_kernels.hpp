@@ -338,18 +338,35 @@ __global__ __launch_bounds__(BLK) void k_cclear(const CohortConst *__restrict__ 
   }
 }
 
-// Copy the level block back into the members' level-major arenas (a block per row range,
-// each thread a fixed set of labels of the row).
+// Copy the level block back into the members' level-major arenas. A block takes a tile of
+// consecutive rows into LDS (one contiguous read), then writes each member's part of the
+// tile as one contiguous run of its level block (vt * C words), so both sides stream.
+constexpr int SCAT_WORDS = 4096;  // LDS words of a tile: vt = SCAT_WORDS / S rows
+static_assert(SCAT_WORDS >= CMAXCH * 64 && CMAXB <= 64, "a scatter tile holds a full row; dbase holds every member");
 __global__ __launch_bounds__(BLK) void k_cscatter(const CohortConst *__restrict__ ccp, const JobConst *__restrict__ jcs,
                                                   GraphDev g) {
   const CohortConst &cc = *ccp;
-  const int L = cc.L, C = cc.C, S = cc.S, nl = cc.B * C;
-  for (int l = threadIdx.x; l < nl; l += BLK) {
-    const int q = l / C, j = l - q * C;
+  const int L = cc.L, C = cc.C, S = cc.S, B = cc.B;
+  __shared__ int32_t tile[SCAT_WORDS];
+  __shared__ int32_t *dbase[64];  // member q's level-L block
+  for (int q = threadIdx.x; q < B; q += BLK) {
     const JobConst &c = jcs[cc.jobs[q]];
-    gi32 *dst = as_global(c.T + c.off[L] + j);
-    const gi32 *src = as_global(cc.CT + l);
-    for (long long v = blockIdx.x; v < g.n; v += gridDim.x) dst[(size_t)v * C] = src[(size_t)v * S];
+    dbase[q] = c.T + c.off[L];
+  }
+  const int VT = SCAT_WORDS / S;  // >= 4: S <= 16 * 64
+  const gi32 *CT = as_global(cc.CT);
+  for (long long v0 = (long long)blockIdx.x * VT; v0 < g.n; v0 += (long long)gridDim.x * VT) {
+    const int nv = (int)min((long long)VT, (long long)g.n - v0);
+    __syncthreads();  // the previous tile has been written out (and dbase is visible)
+    for (int i = threadIdx.x; i < nv * S; i += BLK) tile[i] = CT[(size_t)v0 * S + i];
+    __syncthreads();
+    const int per = nv * C;  // words of one member in this tile
+    const UDiv dper((uint32_t)per), dc((uint32_t)C);
+    for (int i = threadIdx.x; i < B * per; i += BLK) {
+      const int q = (int)dper.div((uint32_t)i), r = i - q * per;  // r = vi * C + j
+      const int vi = (int)dc.div((uint32_t)r), j = r - vi * C;
+      as_global(dbase[q])[(size_t)v0 * C + r] = tile[vi * S + q * C + j];
+    }
   }
 }
 
