@@ -399,7 +399,7 @@ def main():
 
     assert torch.cuda.is_available(), "bench needs a GPU"
     need = args.batch * (args.steps + args.warmup)
-    n_tune = 32  # queries after the batches, used only by the autotuner
+    n_tune = max(32, args.batch)  # queries after the batches, used only by the autotuner
     rng = np.random.default_rng(datasets.SEED_BASE + 77)
     pool = wl.queries[args.k]
     extra = [rng.choice(wl.groups.n_groups, args.k, replace=False).astype(np.int32)
@@ -449,9 +449,11 @@ def main():
 
     # ---- batched path: its own tuning, warmup, then timed steps
     if not args.tuning and not args.no_tune:
-        tuned_b, _ = autotune.tune(eng, tune_q[:min(n_tune, args.batch)],
+        # one batch of the timed step's size per grid point (its cohorts then have the
+        # timed step's size), timed twice
+        tuned_b, _ = autotune.tune(eng, tune_q[:args.batch],
                                    grid={"degree_split": (64, 128, 256, 512), "chunk_edges": (512, 1024, 4096),
-                                         "near_far_delta": (0,)}, budget_s=60.0, batched=True)
+                                         "near_far_delta": (0,)}, repeats=2, budget_s=60.0, batched=True)
         print(f"# autotuned (batched) {tuned_b} ({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
     batches = [queries[i * args.batch:(i + 1) * args.batch] for i in range(args.warmup + args.steps)]
     for b in batches[:args.warmup]:
