@@ -8,8 +8,10 @@ A step = one batch of --batch queries solved by the batched persistent path
 (gst_solve_batch). value = queries/s over the timed steps (inputs resident in HBM).
 Beside it: the plain single-query path on --plain-queries of the same queries (q/s,
 p50/p99), the grow-round roofline (algorithmic bytes / HIP-event time of the select and
-grow kernels of each round, plain path), and the CPU PrunedDP++ restatement timed on the host cores on a
-bounded sample, whose costs also serve as the parity check of the GPU results.
+grow kernels of each round, plain path), the same for the batched step's dominant kernel,
+the cohort's level-1 grow (roofline_batched), and the CPU PrunedDP++ restatement timed on
+the host cores on a bounded sample, whose costs also serve as the parity check of the GPU
+results.
 """
 import argparse
 import json
@@ -459,12 +461,33 @@ def main():
     for b in batches[:args.warmup]:
         eng.solve_batch(b)
     torch.cuda.synchronize()
+    eng.cohort_stats(reset=True)
     t1 = time.perf_counter()
     results = []
     for b in batches[args.warmup:]:
         results += eng.solve_batch(b)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t1
+    # the batched step's dominant kernel: the cohort's level-1 grow (HIP events around
+    # each launch on the batch stream); algorithmic bytes = 8 per arc scanned (adjacency +
+    # weight) + 4 per label relaxation (the target label read)
+    cs = eng.cohort_stats()
+    roof_b = None
+    if cs["grow_launches"] > 0 and cs["grow_device_s"] > 0:
+        cg_bytes = 8 * cs["grow_arcs"] + 4 * cs["grow_relax"]
+        cg_gbs = cg_bytes / cs["grow_device_s"] / 1e9
+        roof_b = {"bound": "hbm", "achieved": round(cg_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                  "frac": round(cg_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                  "kernel": "k_cgrow: cohort level-1 grow (batched path, HIP events on the batch stream)",
+                  "launches": cs["grow_launches"],
+                  "avg_launch_us": round(cs["grow_device_s"] / cs["grow_launches"] * 1e6, 1),
+                  "bytes_per_launch": int(cg_bytes / cs["grow_launches"]),
+                  "bytes_per_unit": "8 per arc scanned + 4 per label relaxation",
+                  "cohorts": cs["cohorts"], "members": cs["members"],
+                  "share_of_step": round(cs["grow_device_s"] / dt, 3)}
+        print(f"# cohort grow: {cs['grow_launches']} launches, {roof_b['avg_launch_us']:.0f} us each, "
+              f"{cg_gbs:.1f} GB/s algorithmic, {roof_b['share_of_step']:.0%} of the timed steps",
+              file=sys.stderr, flush=True)
     nq = sum(len(b) for b in batches[args.warmup:])
     qps = nq / dt
     blat = np.array([r["t_total_s"] for r in results]) * 1e3
@@ -584,6 +607,7 @@ def main():
                      # arcs scanned per second (one random target-state read each), to set
                      # against bench.py --gather-ceiling
                      "arcs_g_per_s": round(arcs_gs, 2)},
+        "roofline_batched": roof_b,
         "cpu_baseline": cpu,
         "mixed_k_batch": mixed,
         "hop": hop,

@@ -128,6 +128,9 @@ struct gst_ctx {
   Plan *d_plan = nullptr;    // work layout of the single-query path
   unsigned long long *d_count = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;  // bracket the work kernels of one round
+  hipEvent_t cev0 = nullptr, cev1 = nullptr;  // bracket one cohort grow launch (batch stream)
+  std::mutex cstat_mu;
+  gst_cohort_stats cstat{};                 // gst_get_cohort_stats
   unsigned merge_grid = 1024;               // resident blocks of k_merge on this device
   unsigned merge_hop_grid = 1024;           // resident blocks of k_merge_hop<false> on this device
   unsigned merge_hopb_grid = 1024;          // resident blocks of k_merge_hop<true>
@@ -687,6 +690,8 @@ int gst_open(const gst_csr *csr, const gst_groups *groups, const gst_opts *opts,
   HIPCHK(hipMalloc((void **)&c->d_plan, sizeof(Plan)));
   HIPCHK(hipEventCreate(&c->ev0));
   HIPCHK(hipEventCreate(&c->ev1));
+  HIPCHK(hipEventCreate(&c->cev0));
+  HIPCHK(hipEventCreate(&c->cev1));
   c->g.beg = c->d_beg; c->g.adj = c->d_adj; c->g.w = c->d_w;
   c->g.goffs = c->d_goffs; c->g.gverts = c->d_gverts; c->g.n = n;
   {
@@ -966,6 +971,14 @@ int gst_info_get(gst_ctx *c, gst_info *info) {
   return GST_OK;
 }
 
+int gst_get_cohort_stats(gst_ctx *c, gst_cohort_stats *out, int32_t reset) {
+  if (!c || !out) return GST_BAD_INPUT;
+  std::lock_guard<std::mutex> lk(c->cstat_mu);
+  *out = c->cstat;
+  if (reset) c->cstat = gst_cohort_stats{};
+  return GST_OK;
+}
+
 void gst_close(gst_ctx *c) {
   if (!c) return;
   (void)hipSetDevice(c->dev);
@@ -981,6 +994,8 @@ void gst_close(gst_ctx *c) {
   (void)hipFree(c->d_plan);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->cev0) (void)hipEventDestroy(c->cev0);
+  if (c->cev1) (void)hipEventDestroy(c->cev1);
   if (c->st) (void)hipStreamDestroy(c->st);
   delete c;
 }

@@ -69,9 +69,14 @@ class GstInfo(ctypes.Structure):
                 ("active_jobs", ctypes.c_int32), ("queued_jobs", ctypes.c_int32), ("completed", ctypes.c_int64)]
 
 
+class GstCohortStats(ctypes.Structure):
+    _fields_ = [("cohorts", ctypes.c_int64), ("members", ctypes.c_int64), ("grow_launches", ctypes.c_int64),
+                ("grow_device_s", ctypes.c_double), ("grow_arcs", ctypes.c_int64), ("grow_relax", ctypes.c_int64)]
+
+
 EXPORTS = ["gst_open", "gst_solve", "gst_solve_batch", "gst_submit", "gst_poll", "gst_wait", "gst_info_get",
            "gst_set_tuning", "gst_result_free", "gst_close", "gst_strerror", "gst_last_error", "gst_arena_bytes",
-           "gst_job_bytes", "gst_solve_noalloc", "gst_set_tuning_k"]
+           "gst_job_bytes", "gst_solve_noalloc", "gst_set_tuning_k", "gst_get_cohort_stats"]
 
 
 def lib_path() -> str:
@@ -107,6 +112,7 @@ def lib():
         L.gst_solve_noalloc.argtypes = [P, P, i32, i32, i32, P, i32, ctypes.POINTER(GstResult)]
         L.gst_job_bytes.restype = ctypes.c_int64
         L.gst_job_bytes.argtypes = [P, i32, i32]
+        L.gst_get_cohort_stats.argtypes = [P, ctypes.POINTER(GstCohortStats), i32]
         _LIB = L
     return _LIB
 
@@ -299,6 +305,15 @@ class Engine:
         if b < 0:
             raise GstError(int(b), "bad query shape")
         return b
+
+    def cohort_stats(self, reset=False) -> dict:
+        """Totals over the cohorts run since open or the last reset: cohorts, members, and
+        their level-1 grow launches, device seconds, arcs scanned and label relaxations."""
+        st = GstCohortStats()
+        rc = lib().gst_get_cohort_stats(self._h, ctypes.byref(st), 1 if reset else 0)
+        if rc != GST_OK:
+            raise GstError(rc, "gst_get_cohort_stats")
+        return {f: getattr(st, f) for f, _ in GstCohortStats._fields_}
 
     def info(self) -> dict:
         i = GstInfo()

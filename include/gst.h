@@ -209,6 +209,19 @@ int gst_solve_noalloc(gst_ctx *ctx, const int32_t *gids, int32_t k, int32_t D, i
  * (arena, activation masks, worklists, tables, tree buffers): what admission charges
  * against dp_budget_bytes. < 0 on bad arguments. */
 int64_t gst_job_bytes(gst_ctx *ctx, int32_t k, int32_t D);
+/* No counterpart in the reference: instrumentation of the batched and served forms'
+ * cohorts (queries of one k whose level 1 grows together in one shared level block).
+ * Totals over every cohort since gst_open or the last reset (reset != 0 clears them after
+ * the copy). */
+typedef struct gst_cohort_stats {
+  int64_t cohorts;        /* cohorts run */
+  int64_t members;        /* queries they held */
+  int64_t grow_launches;  /* level-1 grow launches (one per round) */
+  double grow_device_s;   /* their device time (HIP events around each launch) */
+  int64_t grow_arcs;      /* arcs they scanned (one scan serves every member) */
+  int64_t grow_relax;     /* label relaxations they made */
+} gst_cohort_stats;
+int gst_get_cohort_stats(gst_ctx *ctx, gst_cohort_stats *out, int32_t reset);
 
 #ifdef __cplusplus
 }

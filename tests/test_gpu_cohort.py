@@ -99,3 +99,24 @@ def test_cohort_stats_count_unpruned(ceng, ceng_all, rmat, k, levels):
     for q, r in zip(qs, res):
         assert r["states_finite"] == rmat.graph.n * (2 ** k - 1), (q.gids, r["states_finite"])
         assert r["cost"] == eng.solve(q.gids)["cost"]
+
+
+def test_cohort_instrumentation(rmat):
+    """gst_get_cohort_stats counts the cohorts of a batch, their members and their level-1
+    grow launches, device time, arcs and relaxations; a reset clears the totals. The
+    members' level-1 counters hold an even share of its arc and relaxation totals."""
+    qs = _queries(rmat, [4], 12, 77)
+    with engine.Engine(rmat.graph, rmat.groups, device=0, cohort_min=2) as eng:
+        eng.cohort_stats(reset=True)
+        res = eng.solve_batch(qs)
+        st = eng.cohort_stats(reset=True)
+        print("cohort stats", st, flush=True)
+        assert st["cohorts"] == 1 and st["members"] == len(qs)
+        assert st["grow_launches"] >= 1 and st["grow_device_s"] > 0
+        assert st["grow_arcs"] > 0 and st["grow_relax"] > 0
+        # each member's level-1 counters include an even share of the cohort's level 1
+        assert all(r["level_edges"][1] >= st["grow_arcs"] // len(qs) for r in res)
+        assert all(r["level_relax"][1] >= st["grow_relax"] // len(qs) for r in res)
+        assert eng.cohort_stats()["cohorts"] == 0
+        eng.solve(qs[0].gids)  # the plain path runs no cohort
+        assert eng.cohort_stats()["grow_launches"] == 0
