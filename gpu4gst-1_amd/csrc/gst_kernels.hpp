@@ -2020,27 +2020,8 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(BF ? 4 : 1)
     gi32 *const Tl0 = as_global(c.T) + (size_t)c.off[L] + (size_t)v0 * H * nL;
     gi32 *Tl = Tl0;
     int nitems = tv * nL * H;
-    auto merge_item = [&](int it) {
-      const int row = (int)dnl.div((uint32_t)it);   // vi * H + nd
-      const int j = it - row * nL;
-      const int vi = (int)dh.div((uint32_t)row), nd = row - vi * H;
-      const int p = lsub[L][j];
-      const int low = p & -p, rest = p ^ low;
-      const int b = min(nd, D - nd);
-      const int32_t *pn = pm + vi * RW + nd * WS;  // heights <= nd
-      const int32_t *pb = pm + vi * RW + b * WS;   // heights <= b
-      // splits p1 + p2 = p with low in p1: sm runs over the proper subsets of rest. The
-      // count is the same for every item of the level, so the loop is uniform; an INF
-      // half gives a sum >= INF (2 INF < 2^31), which never wins
-      int m = INF;
-      int sm = rest;
-      for (int t = 1; t < nsplit; t++) {
-        sm = (sm - 1) & rest;
-        const int p1 = low | sm, p2 = p ^ p1;
-        const int s12 = pn[p1] + pb[p2];
-        const int s21 = pb[p1] + pn[p2];
-        m = min(m, min(s12, s21));
-      }
+    // the rest of an item once its minimum over the splits is known
+    auto finish_item = [&](int it, int j, int vi, int nd, int p, int m) {
       bool keep = m < INF;
       if (keep) {
         if (m >= SAT_AT) flag_saturation(s);
@@ -2064,6 +2045,60 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(BF ? 4 : 1)
       merged++;
       if (L < K && nd < D) atomicOr(&amask[(vb + vi) * H + nd], 1ull << j);
     };
+    auto merge_item = [&](int it) {
+      const int row = (int)dnl.div((uint32_t)it);   // vi * H + nd
+      const int j = it - row * nL;
+      const int vi = (int)dh.div((uint32_t)row), nd = row - vi * H;
+      const int p = lsub[L][j];
+      const int low = p & -p, rest = p ^ low;
+      const int b = min(nd, D - nd);
+      const int32_t *pn = pm + vi * RW + nd * WS;  // heights <= nd
+      const int32_t *pb = pm + vi * RW + b * WS;   // heights <= b
+      // splits p1 + p2 = p with low in p1: sm runs over the proper subsets of rest. The
+      // count is the same for every item of the level, so the loop is uniform; an INF
+      // half gives a sum >= INF (2 INF < 2^31), which never wins
+      int m = INF;
+      int sm = rest;
+      for (int t = 1; t < nsplit; t++) {
+        sm = (sm - 1) & rest;
+        const int p1 = low | sm, p2 = p ^ p1;
+        const int s12 = pn[p1] + pb[p2];
+        const int s21 = pb[p1] + pn[p2];
+        m = min(m, min(s12, s21));
+      }
+      finish_item(it, j, vi, nd, p, m);
+    };
+    // The last level has one subset (every group) per (vertex, height), so its tiles hold
+    // few items (k = 7, D = 5: 7 vertices x 6 heights for 256 threads) of 2^(k-1) - 1
+    // splits each: G lanes of a wavefront then share an item, lane r taking the splits
+    // p1 = 1 | (x << 1) with x = r, r + G, ... < 2^(k-1) - 1 (every set with group 0
+    // except the full one), and a minimum across the G lanes.
+    auto merge_full = [&]() {
+      int G = 2;
+      while (G < 16 && nitems * G * 2 <= BLK) G *= 2;
+      const int gsh = __builtin_ctz(G);
+      const int nx = nsplit - 1;
+      const int span = ((nitems * G + WAVE - 1) / WAVE) * WAVE;  // whole wavefronts
+      for (int i = threadIdx.x; i < span; i += BLK) {
+        const int it = i >> gsh, r = i & (G - 1);
+        int m = INF;
+        if (it < nitems) {
+          const int vi = (int)dh.div((uint32_t)it), nd = it - vi * H;  // nL = 1
+          const int b = min(nd, D - nd);
+          const int32_t *pn = pm + vi * RW + nd * WS;
+          const int32_t *pb = pm + vi * RW + b * WS;
+          for (int x = r; x < nx; x += G) {
+            const int p1 = 1 | (x << 1), p2 = c.R ^ p1;
+            m = min(m, min(pn[p1] + pb[p2], pb[p1] + pn[p2]));
+          }
+        }
+        for (int o = G >> 1; o > 0; o >>= 1) {
+          const int y = __shfl_xor(m, o, 64);
+          m = y < m ? y : m;
+        }
+        if (it < nitems && r == 0) finish_item(it, 0, (int)dh.div((uint32_t)it), it - (int)dh.div((uint32_t)it) * H, c.R, m);
+      }
+    };
     if constexpr (!BF) {
       stage();
       __syncthreads();
@@ -2077,7 +2112,9 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(BF ? 4 : 1)
           if (star < ubl) ubl = (int)star;
         }
       }
-      for (int it = threadIdx.x; it < nitems; it += BLK) merge_item(it);
+      if (L == K && nitems * 2 <= BLK) merge_full();
+      else
+        for (int it = threadIdx.x; it < nitems; it += BLK) merge_item(it);
     } else {
       // Bound-first tile (pruning on, 4 <= L < k) of up to tvb vertices. The bound of an
       // item from the singletons alone: the tree of a state (v,p,nd) holds, for each group
