@@ -114,7 +114,7 @@ typedef struct {
   int32_t cohort_max;      /* batched/served forms: queued non-hop queries of one k are
                               grown together through level 1, up to this many at a time
                               (0 = up to 64, as many as fit; 1 = each query on its own) */
-  int32_t cohort_min;      /* fewest queued queries such a cohort is formed with (0 = 32;
+  int32_t cohort_min;      /* fewest queued queries such a cohort is formed with (0 = 16;
                               smaller cohorts hold up the other jobs for less gain) */
   int32_t cohort_levels;   /* levels a cohort grows together before its members go on as
                               separate jobs (0 = 1; capped at k-1) */

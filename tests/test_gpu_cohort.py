@@ -2,9 +2,11 @@
 block (the batched and served forms), then handed to the per-query jobs. Every member's
 result equals the same query solved alone and the CPU oracle; trees are valid; the
 finite-state count pins that every state of every member was written. The engines here
-form cohorts from two queries on (the default waits for 32); `ceng_all` keeps a cohort
+form cohorts from two queries on (the default waits for 16); `ceng_all` keeps a cohort
 together through level k-1 (the merges of its members then write into the cohort's level
 block, and every level but the last comes back from it)."""
+import os
+
 import numpy as np
 import pytest
 
@@ -12,6 +14,7 @@ import pyoracle
 from gpu4gst_1_amd import engine
 
 pytestmark = pytest.mark.gpu
+THREADS = min(os.cpu_count() or 1, 16)
 
 
 @pytest.fixture(scope="module")
@@ -58,8 +61,10 @@ def test_cohort_k5(ceng, ceng_all, rmat, rmat_inst, levels):
 
 @pytest.mark.parametrize("levels", ["1", "all"])
 def test_cohorts_mixed_k_and_hop(ceng, ceng_all, rmat, rmat_inst, levels):
-    """One batch: cohorts of k = 2..7 (k = 7 at up to 29 members per cohort, so 40 of
-    them form two cohorts) beside diameter-bounded queries, which run on their own."""
+    """One batch: cohorts of k = 2..7 beside diameter-bounded queries, which run on their
+    own. The 40 k = 7 queries form one cohort when it grows level 1 only (a row of 7 labels
+    per member), and two when it keeps together through level 6 (rows of up to 35 labels
+    per member: 29 members at most)."""
     eng = ceng if levels == "1" else ceng_all
     qs = _queries(rmat, [2, 3, 4, 6], 5, 7) + _queries(rmat, [7], 40, 8) + _queries(rmat, [3, 5], 4, 9, D=3)
     rng = np.random.default_rng(10)
@@ -69,6 +74,26 @@ def test_cohorts_mixed_k_and_hop(ceng, ceng_all, rmat, rmat_inst, levels):
     res = eng.solve_batch(qs)
     for q, r in zip(qs, res):
         _check(rmat, rmat_inst, q, r, eng.solve(q.gids, D=q.D))
+
+
+def test_cohort_k7_full_width(ceng, rmat, rmat_inst):
+    """64 queries at k = 7: one cohort of 64 members whose level-1 rows take 448 labels, the
+    widest level-1 row (the 8-chunk grow instantiation); every member equals the same
+    query alone and the oracle."""
+    qs = _queries(rmat, [7], 64, 4242)
+    ceng.cohort_stats(reset=True)
+    print("cohort of", len(qs), "k=7 queries", flush=True)
+    res = ceng.solve_batch(qs)
+    st = ceng.cohort_stats(reset=True)
+    assert st["cohorts"] == 1 and st["members"] == len(qs), st
+    exp = pyoracle.batch(rmat_inst, [q.gids for q in qs], algo=0, n_threads=THREADS)
+    for q, r, o in zip(qs, res, exp):
+        alone = ceng.solve(q.gids)
+        assert r["status"] == alone["status"] == engine.GST_OK and o["status"] == 0, (q.gids, r["status"])
+        assert r["cost"] == alone["cost"] == o["tree_cost"], (q.gids, r["cost"], alone["cost"], o["tree_cost"])
+        assert r["tree_cost"] == r["cost"]
+        ok, why = engine.validate_tree(rmat.graph, rmat.groups, q.gids, r["edges"], r["cost"], -1)
+        assert ok, (q.gids, why)
 
 
 def test_cohort_equals_single_jobs(rmat):
