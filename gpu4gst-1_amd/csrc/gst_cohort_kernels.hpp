@@ -298,13 +298,16 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(NWT <= 5 ? 
 // UB candidates a level-1 select pass would offer): a wavefront per vertex, lane q =
 // member q, sum_i T_q(v, {i}) when every label is finite; each lane keeps its minimum and
 // lowers its member's UB once at the end. A label within reach of INF flags its member's
-// saturation check, as the select pass does.
+// saturation check, as the select pass does. Each member's finite level-1 labels are
+// counted into its evaluated-state counter (level 1 has no select pass to count them: a
+// state reached at level 1 counts once, as a best-first search takes it once).
 __global__ __launch_bounds__(BLK) void k_cstar(const CohortConst *__restrict__ ccp, const JobConst *jcs, JobState *jss,
                                                GraphDev g) {
   const CohortConst &cc = *ccp;
   const int lane = lane_id();
   long long best = INF;
   bool sat = false;
+  unsigned long long reached = 0;
   if (lane < cc.B) {
     const gi32 *CT = as_global(cc.CT);
     for (long long v = (long long)blockIdx.x * (BLK / WAVE) + (threadIdx.x >> 6); v < g.n;
@@ -313,6 +316,7 @@ __global__ __launch_bounds__(BLK) void k_cstar(const CohortConst *__restrict__ c
       for (int i = 0; i < cc.K; i++) {
         const int x = CT[(size_t)v * cc.S + lane * cc.C + i];
         sat = sat || (x >= SAT_AT && x < INF);
+        reached += x < INF;
         sum = x >= INF ? (long long)INF : sum + x;
       }
       best = sum < best ? sum : best;
@@ -320,6 +324,7 @@ __global__ __launch_bounds__(BLK) void k_cstar(const CohortConst *__restrict__ c
     JobState *s = &jss[cc.jobs[lane]];
     if (best < INF && !(jcs[cc.jobs[lane]].flags & JF_NO_PRUNE)) atomicMin(&s->ub, (int)best);
     if (sat) flag_saturation(s);
+    if (reached) atomicAdd(jcs[cc.jobs[lane]].stx + (blockIdx.x & (NSTRIPE - 1)) * NSTX + ST_STATES, reached);
   }
 }
 

@@ -114,7 +114,7 @@ typedef struct {
   int32_t cohort_max;      /* batched/served forms: queued non-hop queries of one k are
                               grown together through level 1, up to this many at a time
                               (0 = up to 64, as many as fit; 1 = each query on its own) */
-  int32_t cohort_min;      /* fewest queued queries such a cohort is formed with (0 = 16;
+  int32_t cohort_min;      /* fewest queued queries such a cohort is formed with (0 = 8;
                               smaller cohorts hold up the other jobs for less gain) */
   int32_t cohort_levels;   /* levels a cohort grows together before its members go on as
                               separate jobs (0 = 1; capped at k-1) */
@@ -142,7 +142,8 @@ typedef struct {
   double t_grow_s;        /* device time of the grow kernels (HIP events, plain path) */
   double t_merge_s;       /* device time of the merge kernels (HIP events, plain path) */
   int64_t states_finite;  /* GST_F_STATS: states with a finite cost (the CSV "counts") */
-  int64_t states_processed; /* frontier states evaluated (the CSV "process_num") */
+  int64_t states_processed; /* frontier states evaluated (the CSV "process_num"); level 1
+                               of a cohort member counts each state it reached once */
   int64_t grow_vertices, grow_edges, grow_relax, grow_pushes; /* grow-kernel work counters */
   int64_t arena_bytes;    /* DP arena size V*2^k*(D+1)*4 */
   int64_t level_edges[8]; /* grow work per DP level (subset size 1..k): edges scanned */

@@ -2,7 +2,7 @@
 block (the batched and served forms), then handed to the per-query jobs. Every member's
 result equals the same query solved alone and the CPU oracle; trees are valid; the
 finite-state count pins that every state of every member was written. The engines here
-form cohorts from two queries on (the default waits for 16); `ceng_all` keeps a cohort
+form cohorts from two queries on (the default waits for 8); `ceng_all` keeps a cohort
 together through level k-1 (the merges of its members then write into the cohort's level
 block, and every level but the last comes back from it)."""
 import os

@@ -72,6 +72,25 @@ def test_tiny_graph_batched_equals_plain(tiny):
             assert r["status"] == engine.GST_OK and r["cost"] == cost, (gids, D, r)
 
 
+def test_tiny_graph_cohort_equals_plain(tiny):
+    """The same cases through an engine that groups queued non-hop queries of one k into a
+    cohort from two on: the k = 2 cases (groups in different components, an empty group,
+    shared vertices) then grow their level 1 together."""
+    g, gr, _ = tiny
+    qs = [engine.Query(np.array(gids, np.int32), D) for gids, D, _, _ in CASES]
+    with engine.Engine(g, gr, device=0, cohort_min=2) as ceng:
+        ceng.cohort_stats(reset=True)
+        res = ceng.solve_batch(qs)
+        assert ceng.cohort_stats()["cohorts"] >= 1
+    for (gids, D, cost, _), r in zip(CASES, res):
+        if cost is None:
+            assert r["status"] == engine.GST_NO_SOLUTION, (gids, D, r)
+        else:
+            assert r["status"] == engine.GST_OK and r["cost"] == cost and r["tree_cost"] == cost, (gids, D, r)
+            ok, why = engine.validate_tree(g, gr, np.array(sorted(set(gids)), np.int32), r["edges"], cost, D)
+            assert ok, why
+
+
 @pytest.mark.parametrize("gids, D", [([-1, 0], -1), ([0, 8], -1), ([], -1), (list(range(8)), -1),
                                      ([0, 1], 16)])
 def test_tiny_graph_rejects_bad_queries(tiny, gids, D):
